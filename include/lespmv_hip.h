@@ -1,0 +1,186 @@
+/*
+ * lespmv_hip.h - C ABI of the MI355X (gfx950) LeSpMV SpMV hot path.
+ *
+ * This is the drop-in boundary. The C++ side (include/lespmv/LeSpMV.h) keeps the
+ * reference's struct and function names on top of it; Python reaches it via
+ * ctypes (sspmv_amd/lib.py). Signatures use plain pointers and sizes only.
+ *
+ * Reference interfaces each entry point replaces (paths relative to LeSpMV/):
+ *   lespmv_create_csr      CSR_Matrix<I,V> construction, include/sparse_format.h:95-104,
+ *                          read_csr_matrix, utils/sparse_io.cpp:179-204 (the resulting struct)
+ *   lespmv_destroy         delete_host_matrix / delete_<fmt>_matrix, include/sparse_format.h:300-449
+ *   lespmv_convert         csr_to_coo / csr_to_ell / csr_to_sell / csr_to_sell_c_sigma /
+ *                          csr_to_sell_c_R / csr_to_dia / csr_to_bsr / csr_to_csr5,
+ *                          include/sparse_conversion.h:144-1379
+ *   lespmv_spmv            LeSpMV_csr (src/spmv_csr.cpp:164-185), LeSpMV_coo (src/spmv_coo.cpp:160-182),
+ *                          LeSpMV_ell (src/spmv_ell.cpp:224-247), LeSpMV_sell (src/spmv_s_ell.cpp:202-224),
+ *                          LeSpMV_sell_c_sigma (src/spmv_sell_c_sigma.cpp:189-211),
+ *                          LeSpMV_sell_c_R (src/spmv_sell_c_R.cpp:189-211), LeSpMV_dia
+ *                          (src/spmv_dia.cpp:176-197), LeSpMV_bsr (src/spmv_bsr.cpp:214-233),
+ *                          LeSpMV_csr5 (src/spmv_csr5.cpp:438-457); variant = kernel_flag
+ *   lespmv_spmm            no reference counterpart (multi-vector extension of lespmv_spmv)
+ *   lespmv_export*         the struct fields of include/sparse_format.h:76-295, read back to host
+ *   lespmv_effective_bytes bytes model replacing bytes_per_spmv, include/spmv_benchmark.h:26-150
+ *   lespmv_generate        no reference counterpart (synthetic inputs; the reference reads .mtx)
+ *
+ * Errors: every call returns a status (LESPMV_OK = 0); lespmv_last_error()
+ * gives the message of the last failure on the calling thread. Nothing exits.
+ * Threading: a matrix handle may be used from one thread at a time; different
+ * handles are independent. Every compute call takes a stream (NULL = the
+ * handle's own stream) and is asynchronous unless stated otherwise.
+ */
+#ifndef LESPMV_HIP_H
+#define LESPMV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lespmv_matrix_s* lespmv_matrix_t;
+typedef void* lespmv_stream_t; /* a hipStream_t; NULL selects the handle's stream */
+
+/* Format ids: the reference's label map, LeSpMV/script/write_label.py:63-75. */
+enum {
+  LESPMV_COO = 0,
+  LESPMV_CSR = 1,
+  LESPMV_DIA = 2,
+  LESPMV_ELL = 3,
+  LESPMV_SELL = 4,         /* S-ELL: sliced ELL, rows in natural order */
+  LESPMV_SELL_C_SIGMA = 5,
+  LESPMV_SELL_C_R = 6,
+  LESPMV_CSR5 = 7,
+  LESPMV_BSR = 8,
+  LESPMV_NUM_FORMATS = 9
+};
+
+enum { LESPMV_F32 = 0, LESPMV_F64 = 1 };
+
+enum {
+  LESPMV_OK = 0,
+  LESPMV_ERR_ARG = 1,           /* bad argument or handle */
+  LESPMV_ERR_OOM = 2,           /* device allocation failed */
+  LESPMV_ERR_UNSUPPORTED = 3,   /* shape/parameter not supported (e.g. sigma % C != 0) */
+  LESPMV_ERR_MAX_DIAGS = 4,     /* DIA: more diagonals than max_diags */
+  LESPMV_ERR_DEVICE = 5,        /* HIP runtime error */
+  LESPMV_ERR_NOT_CONVERTED = 6  /* format requested before lespmv_convert */
+};
+
+/* Kernel variants (the GPU meaning of the reference's kernel_flag/kernel_tag). */
+enum {
+  LESPMV_VARIANT_AUTO = 0,      /* the handle's tuned choice */
+  LESPMV_VARIANT_VECTOR = 1,    /* CSR: lanes-per-row vector kernel; DIA: x through caches */
+  LESPMV_VARIANT_MERGE = 2,     /* CSR: merge-path tiles; DIA: LDS-staged x windows */
+  LESPMV_VARIANT_ATOMIC = 3     /* COO: atomics instead of the carry fix-up pass */
+};
+
+/* Conversion parameters. Fields a format does not use are ignored. */
+typedef struct {
+  int32_t chunk;       /* C: rows per chunk (S-ELL, SELL-c-sigma, SELL-c-R) */
+  int32_t sigma;       /* SELL-c-sigma sort window; CSR5 sigma (nonzeros per lane) */
+  int32_t alignment;   /* chunk_len rounding (sliced formats), DIA stride rounding */
+  int32_t ld;          /* ELL layout of the exported arrays: 0 RowMajor, 1 ColMajor */
+  int32_t max_diags;   /* DIA limit (reference MAX_DIAG_NUM = 10240) */
+  int32_t block_rows;  /* BSR br */
+  int32_t block_cols;  /* BSR bc */
+  int32_t omega;       /* CSR5 omega (lanes per tile) */
+} lespmv_params_t;
+
+/* Host-side CSR produced by lespmv_generate; arrays owned by the library. */
+typedef struct {
+  int64_t m, n, nnz;
+  int32_t dtype;
+  int32_t* row_offset;
+  int32_t* col_index;
+  void* values;
+} lespmv_host_csr_t;
+
+/* Synthetic matrix families for lespmv_generate (params as doubles). */
+enum {
+  LESPMV_GEN_LAPLACE2D = 0,   /* nx, ny, diag, off: 5-point natural order */
+  LESPMV_GEN_STENCIL27 = 1,   /* n, diag, off: 27-point on n^3 */
+  LESPMV_GEN_LOGNORMAL = 2,   /* m, n, mean, sigma, cap, frac_local, band */
+  LESPMV_GEN_BLOCKRAND = 3,   /* mb, nb, br, bc, blocks_per_row */
+  LESPMV_GEN_BANDED = 4,      /* m, half_bandwidth, fill */
+  LESPMV_GEN_POWERLAW = 5,    /* m, zipf_exponent, mean_nnz, max_nnz */
+  LESPMV_GEN_BLOCKDIAG = 6,   /* m, block */
+  LESPMV_GEN_UNIFORM = 7      /* m, n, density */
+};
+
+const char* lespmv_last_error(void);
+int lespmv_init(int device);
+int lespmv_device_info(int64_t* hbm_bytes, int32_t* num_cu, int32_t* arch_gfx);
+
+/* Reference (reference_layout=1) or GPU-native (0) default parameters. */
+int lespmv_default_params(int fmt, int dtype, int reference_layout, lespmv_params_t* out);
+
+/* Matrix lifecycle. Arrays are copied into library-owned device memory.
+ * on_device = 1 means the three pointers are device pointers. */
+int lespmv_create_csr(lespmv_matrix_t* out, int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* row_offset,
+                      const int32_t* col_index, const void* values, int on_device);
+int lespmv_destroy(lespmv_matrix_t A);
+int lespmv_matrix_info(lespmv_matrix_t A, int64_t* m, int64_t* n, int64_t* nnz, int32_t* dtype);
+
+/* Build format `fmt` on the device from the handle's CSR (GPU kernels). */
+int lespmv_convert(lespmv_matrix_t A, int fmt, const lespmv_params_t* p);
+int lespmv_release_format(lespmv_matrix_t A, int fmt);
+int lespmv_has_format(lespmv_matrix_t A, int fmt);
+/* Device milliseconds of the last lespmv_convert of `fmt` (hipEvent timed). */
+int lespmv_convert_ms(lespmv_matrix_t A, int fmt, double* ms);
+
+/* y = alpha*A*x + beta*y; x, y are device pointers of the handle's dtype.
+ * beta == 0 never reads y. */
+int lespmv_spmv(lespmv_matrix_t A, int fmt, int variant, double alpha, const void* x, double beta, void* y,
+                lespmv_stream_t stream);
+/* Y = alpha*A*X + beta*Y for k right-hand sides, X row-major N x k (leading
+ * dimension ldx >= k), Y row-major M x k (ldy >= k), device pointers. */
+int lespmv_spmm(lespmv_matrix_t A, int fmt, int variant, int32_t k, double alpha, const void* X, int64_t ldx,
+                double beta, void* Y, int64_t ldy, lespmv_stream_t stream);
+
+/* Average device time in ms of `iters` back-to-back lespmv_spmv calls,
+ * measured with hipEvents on the stream the kernels run on. Synchronous. */
+int lespmv_time_spmv(lespmv_matrix_t A, int fmt, int variant, double alpha, const void* x, double beta, void* y,
+                     int32_t iters, double* ms_per_call);
+int lespmv_time_spmm(lespmv_matrix_t A, int fmt, int variant, int32_t k, double alpha, const void* X, int64_t ldx,
+                     double beta, void* Y, int64_t ldy, int32_t iters, double* ms_per_call);
+
+/* Algorithmic bytes of one SpMV (k = 1) or SpMM with this format: values +
+ * indices + x read once + y written (+ y read if beta != 0). Padding and
+ * repeated x gathers excluded. */
+int lespmv_effective_bytes(lespmv_matrix_t A, int fmt, int32_t k, int beta_nonzero, int64_t* bytes);
+/* The reference's bytes_per_spmv formula (include/spmv_benchmark.h:26-150). */
+int lespmv_reference_bytes(lespmv_matrix_t A, int fmt, int64_t* bytes);
+/* Device bytes held by format `fmt` (LESPMV_CSR: the CSR arrays). */
+int lespmv_format_bytes(lespmv_matrix_t A, int fmt, int64_t* bytes);
+
+/* Read back a format's arrays in the reference's layout (named as the
+ * reference's struct fields, e.g. "reorder", "chunk_len", "col_index",
+ * "tile_desc"; scalars as one int64). Synchronous. */
+int lespmv_export_size(lespmv_matrix_t A, int fmt, const char* name, int64_t* bytes);
+int lespmv_export(lespmv_matrix_t A, int fmt, const char* name, void* host_dst, int64_t bytes);
+
+/* Device memory and stream helpers for callers without their own runtime. */
+int lespmv_malloc(void** dptr, int64_t bytes);
+int lespmv_free(void* dptr);
+int lespmv_memcpy_h2d(void* dst, const void* src, int64_t bytes);
+int lespmv_memcpy_d2h(void* dst, const void* src, int64_t bytes);
+int lespmv_memset(void* dptr, int value, int64_t bytes);
+int lespmv_synchronize(void);
+int lespmv_stream_create(lespmv_stream_t* s);
+int lespmv_stream_destroy(lespmv_stream_t s);
+int lespmv_stream_synchronize(lespmv_stream_t s);
+
+/* Synthetic inputs (host). */
+int lespmv_generate(int family, const double* params, int32_t nparams, uint64_t seed, int dtype,
+                    lespmv_host_csr_t* out);
+int lespmv_host_csr_free(lespmv_host_csr_t* csr);
+/* x[i] = lo + (hi-lo) * u(splitmix64(seed ^ i)), u = (h >> 11) * 2^-53. */
+int lespmv_fill_uniform(void* host_dst, int dtype, int64_t n, uint64_t seed, double lo, double hi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LESPMV_HIP_H */

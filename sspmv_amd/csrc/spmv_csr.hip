@@ -1,0 +1,283 @@
+// CSR SpMV / SpMM kernels for gfx950.
+//
+// Reference semantics: LeSpMV_csr (src/spmv_csr.cpp:164-185) computes
+// y = alpha*A*x + beta*y row by row; kernel_flag 2 balances rows by nonzeros
+// (src/spmv_csr.cpp:137-162, utils/partition.cpp:58-75). On the GPU the two
+// kernels are:
+//  * vector: L lanes of a 64-wide wavefront per row (L = 2..64 tuned from the
+//    row-length distribution), unit-stride loads of col/val, DPP/permute
+//    reduction inside the L-lane group;
+//  * merge: merge-path tiles of 2048 (row end + nonzero) items, so every
+//    workgroup gets the same amount of work whatever the row lengths. The tile's
+//    products are staged in LDS, complete rows are reduced from LDS, and the row
+//    open at a tile edge is carried to a small deterministic fix-up kernel.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace lsp {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMergeItems = 2048;
+
+template <int L, class V>
+__device__ __forceinline__ V group_sum(V v) {
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+  return v;
+}
+
+template <class V>
+__device__ __forceinline__ V axpby(V alpha, V sum, V beta, const V* y) {
+  return beta == V(0) ? alpha * sum : alpha * sum + beta * *y;
+}
+
+template <class V, int L>
+__global__ __launch_bounds__(kBlock) void csr_vector_kernel(int64_t m, const int* __restrict__ rp,
+                                                            const int* __restrict__ ci, const V* __restrict__ val,
+                                                            const V* __restrict__ x, V* __restrict__ y, V alpha,
+                                                            V beta) {
+  const int lane = threadIdx.x & (L - 1);
+  const int64_t row = (int64_t)blockIdx.x * (kBlock / L) + threadIdx.x / L;
+  if (row >= m) return;  // whole L-lane groups leave together
+  const int s = rp[row], e = rp[row + 1];
+  V sum = 0;
+  for (int j = s + lane; j < e; j += L) sum += val[j] * x[ci[j]];
+  sum = group_sum<L>(sum);
+  if (lane == 0) y[row] = axpby(alpha, sum, beta, y + row);
+}
+
+// Number of row ends consumed at merge-path diagonal d (row end r precedes
+// nonzero j iff rp[r+1] <= j).
+__device__ int64_t merge_search(int64_t d, const int* rp, int64_t m, int64_t nnz) {
+  int64_t lo = d - nnz > 0 ? d - nnz : 0, hi = d < m ? d : m;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)rp[mid + 1] <= d - mid - 1) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void merge_partition_kernel(int64_t tiles, int64_t m, int64_t nnz, const int* __restrict__ rp,
+                                       int* __restrict__ coord) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > tiles) return;
+  int64_t d = t * kMergeItems;
+  if (d > m + nnz) d = m + nnz;
+  const int64_t r = merge_search(d, rp, m, nnz);
+  coord[2 * t] = (int)r;
+  coord[2 * t + 1] = (int)(d - r);
+}
+
+template <class V>
+__device__ V block_sum(V v, V* red) {
+  v = group_sum<64>(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  V t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / 64; i++) t += red[i];
+  return t;
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int* __restrict__ rp,
+                                                           const int* __restrict__ ci, const V* __restrict__ val,
+                                                           const V* __restrict__ x, V* __restrict__ y, V alpha,
+                                                           V beta, const int* __restrict__ coord,
+                                                           int* __restrict__ carry_row, V* __restrict__ carry_val,
+                                                           V* __restrict__ head_val) {
+  __shared__ V prod[kMergeItems];
+  __shared__ int rows[kMergeItems + 2];
+  __shared__ V red[kBlock / 64];
+  const int b = blockIdx.x;
+  const int r0 = coord[2 * b], z0 = coord[2 * b + 1];
+  const int r1 = coord[2 * b + 2], z1 = coord[2 * b + 3];
+  const int nz = z1 - z0;
+  const int last = r1 + 1 < m ? r1 + 1 : (int)m;  // rp[r0 .. last] staged
+  for (int i = threadIdx.x; i <= last - r0; i += kBlock) rows[i] = rp[r0 + i];
+  for (int i = threadIdx.x; i < nz; i += kBlock) {
+    const int k = z0 + i;
+    prod[i] = val[k] * x[ci[k]];
+  }
+  __syncthreads();
+  // rows whose end lies in this tile
+  for (int i = threadIdx.x; i < r1 - r0; i += kBlock) {
+    const int s = rows[i], e = rows[i + 1];
+    const int sl = (s > z0 ? s : z0) - z0, el = e - z0;
+    V sum = 0;
+    for (int k = sl; k < el; ++k) sum += prod[k];
+    const int r = r0 + i;
+    if (s >= z0) y[r] = axpby(alpha, sum, beta, y + r);
+    else head_val[b] = sum;  // i == 0: row started in an earlier tile
+  }
+  // the row still open at the tile end
+  if (r1 < m) {
+    const int s = rows[r1 - r0];
+    const int sl = (s > z0 ? s : z0) - z0;
+    V part = 0;
+    for (int k = sl + threadIdx.x; k < nz; k += kBlock) part += prod[k];
+    part = block_sum(part, red);
+    if (threadIdx.x == 0) {
+      carry_row[b] = r1;
+      carry_val[b] = part;
+    }
+  } else if (threadIdx.x == 0) {
+    carry_row[b] = -1;
+  }
+}
+
+// A row that started in an earlier tile and ends in tile b: add the partials of
+// the preceding tiles that left it open, in tile order from the nearest.
+template <class V>
+__global__ void csr_merge_fixup_kernel(int64_t tiles, const int* __restrict__ coord, const int* __restrict__ rp,
+                                       const int* __restrict__ carry_row, const V* __restrict__ carry_val,
+                                       const V* __restrict__ head_val, V* __restrict__ y, V alpha, V beta) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= tiles) return;
+  const int r0 = coord[2 * b], z0 = coord[2 * b + 1], r1 = coord[2 * b + 2];
+  if (!(r0 < r1 && rp[r0] < z0)) return;
+  V total = head_val[b];
+  for (int64_t t = b - 1; t >= 0 && carry_row[t] == r0; --t) total += carry_val[t];
+  y[r0] = axpby(alpha, total, beta, y + r0);
+}
+
+// SpMM: Y(M x k) = alpha*A*X(N x k) + beta*Y, row-major X/Y. A group of KP
+// lanes (KP = k rounded up to a power of two) owns one row; lane c owns
+// column c, so each gathered X row is one contiguous load of k values.
+template <class V, int KP>
+__global__ __launch_bounds__(kBlock) void csr_spmm_kernel(int64_t m, int k, const int* __restrict__ rp,
+                                                          const int* __restrict__ ci, const V* __restrict__ val,
+                                                          const V* __restrict__ X, int64_t ldx,
+                                                          V* __restrict__ Y, int64_t ldy, V alpha, V beta) {
+  const int c = threadIdx.x & (KP - 1);
+  const int64_t row = (int64_t)blockIdx.x * (kBlock / KP) + threadIdx.x / KP;
+  if (row >= m || c >= k) return;
+  const int s = rp[row], e = rp[row + 1];
+  V sum = 0;
+  int j = s;
+  for (; j + 1 < e; j += 2) {
+    const int c0 = ci[j], c1 = ci[j + 1];
+    const V v0 = val[j], v1 = val[j + 1];
+    sum += v0 * X[(int64_t)c0 * ldx + c];
+    sum += v1 * X[(int64_t)c1 * ldx + c];
+  }
+  if (j < e) sum += val[j] * X[(int64_t)ci[j] * ldx + c];
+  V* yp = Y + row * ldy + c;
+  *yp = axpby(alpha, sum, beta, yp);
+}
+
+template <class V, int L>
+void launch_vector(Matrix& A, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  const int64_t rows_per_block = kBlock / L;
+  hipLaunchKernelGGL((csr_vector_kernel<V, L>), grid1d(A.m, (int)rows_per_block), dim3(kBlock), 0, s, A.m,
+                     A.rp.as<int>(), A.ci.as<int>(), A.val.as<V>(), static_cast<const V*>(x), static_cast<V*>(y),
+                     (V)alpha, (V)beta);
+  LSP_LAUNCHED();
+}
+
+template <class V>
+void vector_dispatch(Matrix& A, int lanes, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  switch (lanes) {
+    case 2: launch_vector<V, 2>(A, alpha, x, beta, y, s); break;
+    case 4: launch_vector<V, 4>(A, alpha, x, beta, y, s); break;
+    case 8: launch_vector<V, 8>(A, alpha, x, beta, y, s); break;
+    case 16: launch_vector<V, 16>(A, alpha, x, beta, y, s); break;
+    case 32: launch_vector<V, 32>(A, alpha, x, beta, y, s); break;
+    default: launch_vector<V, 64>(A, alpha, x, beta, y, s); break;
+  }
+}
+
+template <class V>
+void merge_run(Matrix& A, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  CsrMeta& mt = A.meta;
+  const int64_t tiles = mt.merge_tiles;
+  if (tiles == 0) return;
+  hipLaunchKernelGGL((csr_merge_kernel<V>), dim3((unsigned)tiles), dim3(kBlock), 0, s, A.m, A.rp.as<int>(),
+                     A.ci.as<int>(), A.val.as<V>(), static_cast<const V*>(x), static_cast<V*>(y), (V)alpha,
+                     (V)beta, mt.merge_coord.as<int>(), mt.carry_row.as<int>(), mt.carry_val.as<V>(),
+                     mt.head_val.as<V>());
+  LSP_LAUNCHED();
+  hipLaunchKernelGGL((csr_merge_fixup_kernel<V>), grid1d(tiles, kBlock), dim3(kBlock), 0, s, tiles,
+                     mt.merge_coord.as<int>(), A.rp.as<int>(), mt.carry_row.as<int>(), mt.carry_val.as<V>(),
+                     mt.head_val.as<V>(), static_cast<V*>(y), (V)alpha, (V)beta);
+  LSP_LAUNCHED();
+}
+
+template <class V, int KP>
+void launch_spmm(Matrix& A, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
+                 hipStream_t s) {
+  hipLaunchKernelGGL((csr_spmm_kernel<V, KP>), grid1d(A.m, kBlock / KP), dim3(kBlock), 0, s, A.m, k,
+                     A.rp.as<int>(), A.ci.as<int>(), A.val.as<V>(), static_cast<const V*>(X), ldx,
+                     static_cast<V*>(Y), ldy, (V)alpha, (V)beta);
+  LSP_LAUNCHED();
+}
+
+template <class V>
+void spmm_dispatch(Matrix& A, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
+                   hipStream_t s) {
+  if (k <= 1) launch_spmm<V, 1>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 2) launch_spmm<V, 2>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 4) launch_spmm<V, 4>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 8) launch_spmm<V, 8>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 16) launch_spmm<V, 16>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 32) launch_spmm<V, 32>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+  else launch_spmm<V, 64>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+}
+
+int lanes_for(double avg) {
+  int L = 2;
+  while (L < 64 && L < avg) L *= 2;
+  return L;
+}
+
+}  // namespace
+
+void csr_analyse(Matrix& A, hipStream_t s) {
+  CsrMeta& mt = A.meta;
+  mt.max_row = A.m ? reduce_max_rowlen(A.rp.as<int32_t>(), A.m, s) : 0;
+  mt.avg_row = A.m ? (double)A.nnz / (double)A.m : 0.0;
+  mt.lanes = lanes_for(mt.avg_row);
+  // merge path is the default once rows are long-tailed enough that a wave of
+  // the vector kernel would wait on its longest row
+  mt.auto_variant = (mt.max_row > 8 * std::max<double>(mt.avg_row, 4.0)) ? LESPMV_VARIANT_MERGE : LESPMV_VARIANT_VECTOR;
+  const int64_t items = A.m + A.nnz;
+  mt.merge_tiles = ceil_div(items, kMergeItems);
+  mt.merge_coord.alloc(sizeof(int) * 2 * (mt.merge_tiles + 1));
+  mt.carry_row.alloc(sizeof(int) * std::max<int64_t>(mt.merge_tiles, 1));
+  mt.carry_val.alloc(vsize(A.dtype) * std::max<int64_t>(mt.merge_tiles, 1));
+  mt.head_val.alloc(vsize(A.dtype) * std::max<int64_t>(mt.merge_tiles, 1));
+  hipLaunchKernelGGL(merge_partition_kernel, grid1d(mt.merge_tiles + 1, kBlock), dim3(kBlock), 0, s, mt.merge_tiles,
+                     A.m, A.nnz, A.rp.as<int>(), mt.merge_coord.as<int>());
+  LSP_LAUNCHED();
+  mt.analysed = true;
+}
+
+void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  if (A.m == 0) return;
+  if (!A.meta.analysed) csr_analyse(A, s);
+  if (variant == LESPMV_VARIANT_AUTO) variant = A.meta.auto_variant;
+  if (variant == LESPMV_VARIANT_MERGE) {
+    if (A.dtype == LESPMV_F64) merge_run<double>(A, alpha, x, beta, y, s);
+    else merge_run<float>(A, alpha, x, beta, y, s);
+    return;
+  }
+  // vector kernel; variants >= 16 request an explicit lane count (16 + log2 L)
+  int lanes = A.meta.lanes;
+  if (variant >= 16 && variant <= 22) lanes = 1 << (variant - 16);
+  if (lanes < 2) lanes = 2;
+  if (A.dtype == LESPMV_F64) vector_dispatch<double>(A, lanes, alpha, x, beta, y, s);
+  else vector_dispatch<float>(A, lanes, alpha, x, beta, y, s);
+}
+
+void csr_spmm(Matrix& A, int variant, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
+              int64_t ldy, hipStream_t s) {
+  (void)variant;
+  if (A.m == 0 || k == 0) return;
+  if (A.dtype == LESPMV_F64) spmm_dispatch<double>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+  else spmm_dispatch<float>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+}
+
+}  // namespace lsp

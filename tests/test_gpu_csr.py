@@ -1,0 +1,122 @@
+"""CSR SpMV / SpMM on the GPU through the C ABI, against the CPU oracle.
+
+Tolerances (relative 2-norm vs the FP64 oracle on the same inputs): 1e-12 for
+FP64, 1e-5 for FP32; each FP32 case also records its measured error.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import CSR
+from tests.conftest import TOL, rel2
+from tests.matrices import HostCsr, random_csr, suite
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [0, 1, 2, 17, 18, 19, 20, 21, 22]  # auto, vector(tuned), merge, vector L=2..64
+
+
+def oracle_y(oracle, A, alpha, x, beta, y0):
+    y = y0.astype(np.float64).copy()
+    oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v.astype(np.float64), alpha, x.astype(np.float64), beta, y)
+    return y
+
+
+def gpu_y(gpu, M, fmt, alpha, x, beta, y0, variant=0):
+    dx = gpu.DeviceArray.from_numpy(x)
+    dy = gpu.DeviceArray.from_numpy(y0)
+    M.spmv(fmt, alpha, dx, beta, dy, variant=variant)
+    return dy.numpy()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_csr_suite_all_variants(gpu, oracle, dt):
+    rng = np.random.default_rng(11)
+    for name, A in suite(dt).items():
+        M = gpu.Matrix.from_host(A)
+        x = rng.uniform(-1, 1, A.n).astype(dt)
+        y0 = rng.uniform(-1, 1, A.m).astype(dt)
+        ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+        ref0 = oracle_y(oracle, A, 1.0, x, 0.0, y0)
+        for var in VARIANTS:
+            y = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
+            assert rel2(y, ref) <= TOL[np.dtype(dt)], (name, var, rel2(y, ref))
+            # beta = 0 must not read y: start from NaN
+            y = gpu_y(gpu, M, CSR, 1.0, x, 0.0, np.full(A.m, np.nan, dt), var)
+            assert np.all(np.isfinite(y)), (name, var)
+            assert rel2(y, ref0) <= TOL[np.dtype(dt)], (name, var)
+
+
+def test_csr_matches_reference_fixture(gpu, golden):
+    """y of the compiled reference's LeSpMV_csr (flag 0, alpha .8, beta .7, y0 0)."""
+    names = sorted({k.split("/")[1] for k in golden.files if k.startswith("float64/") and k.endswith("/m")})
+    for dn in ("float64", "float32"):
+        for name in names:
+            if f"{dn}/{name}/m" not in golden.files:
+                continue
+            g = lambda s: golden[f"{dn}/{name}/{s}"]
+            A = HostCsr(int(g("m")), int(g("n")), g("rp"), g("ci"), g("v"))
+            M = gpu.Matrix.from_host(A)
+            for var in (1, 2):
+                y = gpu_y(gpu, M, CSR, 0.8, g("x"), 0.7, np.zeros(A.m, A.v.dtype), var)
+                assert rel2(y, g("csr/0/y")) <= TOL[A.v.dtype], (dn, name, var)
+
+
+def test_csr_edge_shapes(gpu, oracle):
+    rng = np.random.default_rng(3)
+    cases = {
+        "all_empty": HostCsr(100, 50, np.zeros(101, np.int32), np.zeros(0, np.int32), np.zeros(0)),
+        "one_by_one": HostCsr(1, 1, np.array([0, 1], np.int32), np.array([0], np.int32), np.array([2.5])),
+        "huge_row": random_csr(40, 120000, seed=5, mean=3.0, long_rows=2, long_len=100000),
+        "many_empty": random_csr(30000, 30000, seed=6, mean=1.0, empty_frac=0.9),
+    }
+    for name, A in cases.items():
+        M = gpu.Matrix.from_host(A)
+        x = rng.uniform(-1, 1, A.n)
+        y0 = rng.uniform(-1, 1, A.m)
+        ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+        for var in (1, 2, 22):
+            y = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
+            assert rel2(y, ref) <= 1e-12, (name, var)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_csr_generated_configs(gpu, oracle, dt):
+    """Config-1 Laplacian (1024^2) and a 300k-row lognormal matrix of the config-2 family."""
+    for fam, prm, seed in ((gpu.GEN_LAPLACE2D, [1024, 1024, 4.0, -1.0], 0),
+                           (gpu.GEN_LOGNORMAL, [300000, 300000, 16.0, 1.0, 4096, 0.3, 64], 2)):
+        A = gpu.generate(fam, prm, seed, dt)
+        M = gpu.Matrix.from_host(A)
+        x = gpu.fill_uniform(A.n, 4, dt)
+        y0 = gpu.fill_uniform(A.m, 9, dt)
+        ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+        for var in (0, 1, 2):
+            y = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
+            err = rel2(y, ref)
+            print(f"{np.dtype(dt).name} fam={fam} variant={var} rel2={err:.3e}")
+            assert err <= TOL[np.dtype(dt)]
+
+
+def test_laplacian_closed_form(gpu):
+    """5-point Laplacian times the constant vector vanishes on interior rows and
+    equals the number of missing neighbours on the boundary."""
+    A = gpu.generate(gpu.GEN_LAPLACE2D, [257, 129, 4.0, -1.0], 0)
+    M = gpu.Matrix.from_host(A)
+    for var in (1, 2):
+        y = gpu_y(gpu, M, CSR, 1.0, np.ones(A.n), 0.0, np.zeros(A.m), var)
+        missing = 5 - np.diff(A.rp)
+        assert np.array_equal(y, missing.astype(np.float64))
+
+
+@pytest.mark.parametrize("k", [1, 4, 8, 16, 32])
+def test_csr_spmm_columns_equal_spmv(gpu, oracle, k):
+    rng = np.random.default_rng(k)
+    A = random_csr(3000, 2500, seed=k, mean=9.0, long_rows=3, long_len=900)
+    M = gpu.Matrix.from_host(A)
+    X = rng.uniform(-1, 1, (A.n, k))
+    Y0 = rng.uniform(-1, 1, (A.m, k))
+    dX, dY = gpu.DeviceArray.from_numpy(X), gpu.DeviceArray.from_numpy(Y0)
+    M.spmm(CSR, k, 0.8, dX, k, 0.7, dY, k)
+    Y = dY.numpy()
+    for j in range(k):
+        ref = oracle_y(oracle, A, 0.8, X[:, j].copy(), 0.7, Y0[:, j].copy())
+        assert rel2(Y[:, j], ref) <= 1e-12, j
