@@ -42,9 +42,16 @@ __global__ __launch_bounds__(kBlock) void csr_vector_kernel(int64_t m, const int
   const int64_t row = (int64_t)blockIdx.x * (kBlock / L) + threadIdx.x / L;
   if (row >= m) return;  // whole L-lane groups leave together
   const int s = rp[row], e = rp[row + 1];
-  V sum = 0;
-  for (int j = s + lane; j < e; j += L) sum += val[j] * x[ci[j]];
-  sum = group_sum<L>(sum);
+  V sum = 0, sum2 = 0;
+  int j = s + lane;
+  for (; j + 3 * L < e; j += 4 * L) {
+    const int c0 = ci[j], c1 = ci[j + L], c2 = ci[j + 2 * L], c3 = ci[j + 3 * L];
+    const V v0 = val[j], v1 = val[j + L], v2 = val[j + 2 * L], v3 = val[j + 3 * L];
+    sum += v0 * x[c0] + v1 * x[c1];
+    sum2 += v2 * x[c2] + v3 * x[c3];
+  }
+  for (; j < e; j += L) sum += val[j] * x[ci[j]];
+  sum = group_sum<L>(sum + sum2);
   if (lane == 0) y[row] = axpby(alpha, sum, beta, y + row);
 }
 
@@ -82,6 +89,9 @@ __device__ V block_sum(V v, V* red) {
   return t;
 }
 
+constexpr int kItems = kMergeItems / kBlock;  // nonzeros staged per thread
+constexpr int kLongRow = 48;                     // rows longer than this in a tile are reduced by a wavefront
+
 template <class V>
 __global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int* __restrict__ rp,
                                                            const int* __restrict__ ci, const V* __restrict__ val,
@@ -92,39 +102,76 @@ __global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int*
   __shared__ V prod[kMergeItems];
   __shared__ int rows[kMergeItems + 2];
   __shared__ V red[kBlock / 64];
+  __shared__ int long_rows[kMergeItems / kLongRow + 2];
+  __shared__ int nlong;
   const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = coord[2 * b], z0 = coord[2 * b + 1];
   const int r1 = coord[2 * b + 2], z1 = coord[2 * b + 3];
   const int nz = z1 - z0;
   const int last = r1 + 1 < m ? r1 + 1 : (int)m;  // rp[r0 .. last] staged
-  for (int i = threadIdx.x; i <= last - r0; i += kBlock) rows[i] = rp[r0 + i];
-  for (int i = threadIdx.x; i < nz; i += kBlock) {
-    const int k = z0 + i;
-    prod[i] = val[k] * x[ci[k]];
+  if (tid == 0) nlong = 0;
+  // every load of the tile is issued before the first gather returns
+  int cidx[kItems];
+  V v[kItems];
+#pragma unroll
+  for (int q = 0; q < kItems; q++) {
+    const int i = tid + q * kBlock;
+    const bool ok = i < nz;
+    cidx[q] = ok ? ci[z0 + i] : 0;
+    v[q] = ok ? val[z0 + i] : V(0);
+  }
+  for (int i = tid; i <= last - r0; i += kBlock) rows[i] = rp[r0 + i];
+  V xv[kItems];
+#pragma unroll
+  for (int q = 0; q < kItems; q++) xv[q] = (tid + q * kBlock < nz) ? x[cidx[q]] : V(0);
+#pragma unroll
+  for (int q = 0; q < kItems; q++) {
+    const int i = tid + q * kBlock;
+    if (i < nz) prod[i] = v[q] * xv[q];
   }
   __syncthreads();
-  // rows whose end lies in this tile
-  for (int i = threadIdx.x; i < r1 - r0; i += kBlock) {
+  // rows whose end lies in this tile: short ones by one thread each
+  for (int i = tid; i < r1 - r0; i += kBlock) {
     const int s = rows[i], e = rows[i + 1];
     const int sl = (s > z0 ? s : z0) - z0, el = e - z0;
+    if (el - sl > kLongRow) {
+      long_rows[atomicAdd(&nlong, 1)] = i;
+      continue;
+    }
     V sum = 0;
     for (int k = sl; k < el; ++k) sum += prod[k];
     const int r = r0 + i;
     if (s >= z0) y[r] = axpby(alpha, sum, beta, y + r);
     else head_val[b] = sum;  // i == 0: row started in an earlier tile
   }
+  __syncthreads();
+  // long rows: one wavefront each
+  for (int q = wave; q < nlong; q += kBlock / 64) {
+    const int i = long_rows[q];
+    const int s = rows[i], e = rows[i + 1];
+    const int sl = (s > z0 ? s : z0) - z0, el = e - z0;
+    V sum = 0;
+    for (int k = sl + lane; k < el; k += 64) sum += prod[k];
+    sum = group_sum<64>(sum);
+    if (lane == 0) {
+      const int r = r0 + i;
+      if (s >= z0) y[r] = axpby(alpha, sum, beta, y + r);
+      else head_val[b] = sum;
+    }
+  }
   // the row still open at the tile end
   if (r1 < m) {
     const int s = rows[r1 - r0];
     const int sl = (s > z0 ? s : z0) - z0;
     V part = 0;
-    for (int k = sl + threadIdx.x; k < nz; k += kBlock) part += prod[k];
+    for (int k = sl + tid; k < nz; k += kBlock) part += prod[k];
     part = block_sum(part, red);
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       carry_row[b] = r1;
       carry_val[b] = part;
     }
-  } else if (threadIdx.x == 0) {
+  } else if (tid == 0) {
     carry_row[b] = -1;
   }
 }
@@ -242,7 +289,10 @@ void csr_analyse(Matrix& A, hipStream_t s) {
   mt.lanes = lanes_for(mt.avg_row);
   // merge path is the default once rows are long-tailed enough that a wave of
   // the vector kernel would wait on its longest row
-  mt.auto_variant = (mt.max_row > 8 * std::max<double>(mt.avg_row, 4.0)) ? LESPMV_VARIANT_MERGE : LESPMV_VARIANT_VECTOR;
+  // the merge path bounds every workgroup's work; the vector kernel only wins
+  // when rows are short and uniform (a wave then covers whole rows at once)
+  mt.auto_variant = (mt.max_row > 2 * std::max<double>(mt.avg_row, 4.0) || mt.avg_row > 32) ? LESPMV_VARIANT_MERGE
+                                                                                            : LESPMV_VARIANT_VECTOR;
   const int64_t items = A.m + A.nnz;
   mt.merge_tiles = ceil_div(items, kMergeItems);
   mt.merge_coord.alloc(sizeof(int) * 2 * (mt.merge_tiles + 1));

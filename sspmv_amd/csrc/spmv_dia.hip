@@ -1,0 +1,304 @@
+// DIA on gfx950.
+//
+// Reference: csr_to_dia (include/sparse_conversion.h:611-766) numbers the
+// occupied diagonals in order of first appearance in a row-major scan, stores
+// offsets col-row and data[d*stride + row] with stride = M rounded up to
+// `alignment`, and refuses more than max_diags diagonals. LeSpMV_dia
+// (src/spmv_dia.cpp:176-197) accumulates diagonal by diagonal.
+//
+// GPU conversion: atomicMin of the first nonzero index per diagonal, a radix
+// sort of the occupied diagonals by that index (= first-appearance order),
+// then a thread-per-row fill that keeps the reference's last-duplicate-wins.
+// GPU kernels own rows instead of diagonals, so there are no atomics:
+//  * vector (variant 1): one thread per row, x through L1/L2;
+//  * window (variant 2): a workgroup owns 256 rows and stages the x windows it
+//    needs in LDS, one window per cluster of nearby offsets (a 27-point stencil
+//    has three clusters); falls back to variant 1 when the windows do not fit.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace lsp {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxClusters = 16;
+constexpr int kLdsBytes = 48 * 1024;
+
+__global__ void diag_first(int64_t m, const int* __restrict__ rp, const int* __restrict__ ci,
+                           int* __restrict__ first) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int j = rp[i]; j < rp[i + 1]; j++) atomicMin(first + (m - i) + ci[j], j);
+}
+
+__global__ void diag_flags(int64_t len, const int* __restrict__ first, int* __restrict__ flag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < len) flag[k] = first[k] != INT32_MAX;
+}
+
+__global__ void diag_compact(int64_t len, const int* __restrict__ first, const int64_t* __restrict__ pos,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= len || first[k] == INT32_MAX) return;
+  keys[pos[k]] = (uint32_t)first[k];
+  idx[pos[k]] = (uint32_t)k;
+}
+
+__global__ void diag_rank(int64_t nd, int64_t m, const uint32_t* __restrict__ idx, int* __restrict__ rank,
+                          int64_t* __restrict__ offs) {
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nd) return;
+  rank[idx[d]] = (int)d;
+  offs[d] = (int64_t)idx[d] - m;
+}
+
+template <class V>
+__global__ void dia_fill(int64_t m, const int* __restrict__ rp, const int* __restrict__ ci, const V* __restrict__ val,
+                         const int* __restrict__ rank, int64_t stride, V* __restrict__ data) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int j = rp[i]; j < rp[i + 1]; j++) data[(int64_t)rank[(m - i) + ci[j]] * stride + i] = val[j];
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void dia_vector(int64_t m, int64_t n, int nd, int64_t stride,
+                                                     const int64_t* __restrict__ offs, const V* __restrict__ data,
+                                                     const V* __restrict__ x, V* __restrict__ y, V alpha, V beta) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= m) return;
+  V sum = 0;
+  for (int d = 0; d < nd; d++) {
+    const int64_t j = i + offs[d];
+    if (j >= 0 && j < n) sum += data[(int64_t)d * stride + i] * x[j];
+  }
+  y[i] = beta == V(0) ? alpha * sum : alpha * sum + beta * y[i];
+}
+
+struct Cluster {
+  int64_t lo, hi;  // offset range
+  int base;        // LDS element base of this cluster's window
+};
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void dia_window(int64_t m, int64_t n, int nd, int64_t stride,
+                                                     const int64_t* __restrict__ offs,
+                                                     const int* __restrict__ lds_off, const V* __restrict__ data,
+                                                     const V* __restrict__ x, V* __restrict__ y, V alpha, V beta,
+                                                     int nclusters, const int64_t* __restrict__ clo,
+                                                     const int* __restrict__ cwidth, const int* __restrict__ cbase) {
+  extern __shared__ unsigned char smem[];
+  V* win = reinterpret_cast<V*>(smem);
+  const int64_t r0 = (int64_t)blockIdx.x * kBlock;
+  for (int c = 0; c < nclusters; c++) {
+    const int64_t start = r0 + clo[c];
+    const int w = cwidth[c], base = cbase[c];
+    for (int t = threadIdx.x; t < w; t += kBlock) {
+      const int64_t j = start + t;
+      win[base + t] = (j >= 0 && j < n) ? x[j] : V(0);
+    }
+  }
+  __syncthreads();
+  const int64_t i = r0 + threadIdx.x;
+  if (i >= m) return;
+  V sum = 0;
+  for (int d = 0; d < nd; d++) {
+    const int64_t j = i + offs[d];
+    if (j >= 0 && j < n) sum += data[(int64_t)d * stride + i] * win[lds_off[d] + threadIdx.x];
+  }
+  y[i] = beta == V(0) ? alpha * sum : alpha * sum + beta * y[i];
+}
+
+template <class V, int KP>
+__global__ __launch_bounds__(kBlock) void dia_spmm_kernel(int64_t m, int64_t n, int k, int nd, int64_t stride,
+                                                          const int64_t* __restrict__ offs,
+                                                          const V* __restrict__ data, const V* __restrict__ X,
+                                                          int64_t ldx, V* __restrict__ Y, int64_t ldy, V alpha,
+                                                          V beta) {
+  const int c = threadIdx.x & (KP - 1);
+  const int64_t i = (int64_t)blockIdx.x * (kBlock / KP) + threadIdx.x / KP;
+  if (i >= m || c >= k) return;
+  V sum = 0;
+  for (int d = 0; d < nd; d++) {
+    const int64_t j = i + offs[d];
+    if (j >= 0 && j < n) sum += data[(int64_t)d * stride + i] * X[j * ldx + c];
+  }
+  V* yp = Y + i * ldy + c;
+  *yp = beta == V(0) ? alpha * sum : alpha * sum + beta * *yp;
+}
+
+struct DiaDev : DiaData {
+  // LDS window plan (variant 2)
+  int nclusters = 0;
+  int lds_elems = 0;
+  DBuf lds_off, clo, cwidth, cbase;
+  int64_t device_bytes() const override {
+    return offsets.bytes + data.bytes + lds_off.bytes + clo.bytes + cwidth.bytes + cbase.bytes;
+  }
+};
+
+int bits_for(uint64_t v) {
+  int b = 0;
+  while (b < 64 && (v >> b)) b++;
+  return b;
+}
+
+// Group sorted offsets into clusters whose windows fit the LDS budget.
+void plan_windows(DiaDev& d, const std::vector<int64_t>& offs, size_t vbytes, hipStream_t s) {
+  d.nclusters = 0;
+  if (offs.empty()) return;
+  std::vector<int64_t> sorted(offs);
+  std::sort(sorted.begin(), sorted.end());
+  std::vector<std::pair<int64_t, int64_t>> cl;
+  for (int64_t o : sorted) {
+    if (!cl.empty() && o - cl.back().second <= kBlock) cl.back().second = o;
+    else cl.push_back({o, o});
+  }
+  if ((int)cl.size() > kMaxClusters) return;
+  std::vector<int64_t> lo;
+  std::vector<int> width, base;
+  int total = 0;
+  for (auto& c : cl) {
+    lo.push_back(c.first);
+    width.push_back((int)(c.second - c.first + kBlock));
+    base.push_back(total);
+    total += width.back();
+  }
+  if ((size_t)total * vbytes > (size_t)kLdsBytes) return;
+  std::vector<int> loff(offs.size());
+  for (size_t q = 0; q < offs.size(); q++) {
+    for (size_t c = 0; c < cl.size(); c++)
+      if (offs[q] >= cl[c].first && offs[q] <= cl[c].second) loff[q] = base[c] + (int)(offs[q] - cl[c].first);
+  }
+  d.lds_off.alloc(sizeof(int) * loff.size());
+  d.clo.alloc(sizeof(int64_t) * lo.size());
+  d.cwidth.alloc(sizeof(int) * width.size());
+  d.cbase.alloc(sizeof(int) * base.size());
+  LSP_HIP(hipMemcpyAsync(d.lds_off.p, loff.data(), sizeof(int) * loff.size(), hipMemcpyHostToDevice, s));
+  LSP_HIP(hipMemcpyAsync(d.clo.p, lo.data(), sizeof(int64_t) * lo.size(), hipMemcpyHostToDevice, s));
+  LSP_HIP(hipMemcpyAsync(d.cwidth.p, width.data(), sizeof(int) * width.size(), hipMemcpyHostToDevice, s));
+  LSP_HIP(hipMemcpyAsync(d.cbase.p, base.data(), sizeof(int) * base.size(), hipMemcpyHostToDevice, s));
+  LSP_HIP(hipStreamSynchronize(s));
+  d.nclusters = (int)cl.size();
+  d.lds_elems = total;
+}
+
+template <class V>
+void run(Matrix& A, DiaDev& d, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  if (variant == LESPMV_VARIANT_MERGE || (variant == LESPMV_VARIANT_AUTO && d.nclusters > 0)) {
+    if (d.nclusters > 0) {
+      hipLaunchKernelGGL((dia_window<V>), grid1d(A.m, kBlock), dim3(kBlock), d.lds_elems * sizeof(V), s, A.m, A.n,
+                         d.ndiag, d.stride, d.offsets.as<int64_t>(), d.lds_off.as<int>(), d.data.as<V>(),
+                         static_cast<const V*>(x), static_cast<V*>(y), (V)alpha, (V)beta, d.nclusters,
+                         d.clo.as<int64_t>(), d.cwidth.as<int>(), d.cbase.as<int>());
+      LSP_LAUNCHED();
+      return;
+    }
+  }
+  hipLaunchKernelGGL((dia_vector<V>), grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, A.n, d.ndiag, d.stride,
+                     d.offsets.as<int64_t>(), d.data.as<V>(), static_cast<const V*>(x), static_cast<V*>(y),
+                     (V)alpha, (V)beta);
+  LSP_LAUNCHED();
+}
+
+template <class V, int KP>
+void spmm_launch(Matrix& A, DiaDev& d, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
+                 int64_t ldy, hipStream_t s) {
+  hipLaunchKernelGGL((dia_spmm_kernel<V, KP>), grid1d(A.m, kBlock / KP), dim3(kBlock), 0, s, A.m, A.n, k, d.ndiag,
+                     d.stride, d.offsets.as<int64_t>(), d.data.as<V>(), static_cast<const V*>(X), ldx,
+                     static_cast<V*>(Y), ldy, (V)alpha, (V)beta);
+  LSP_LAUNCHED();
+}
+
+template <class V>
+void spmm_dispatch(Matrix& A, DiaDev& d, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
+                   int64_t ldy, hipStream_t s) {
+  if (k <= 1) spmm_launch<V, 1>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 2) spmm_launch<V, 2>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 4) spmm_launch<V, 4>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 8) spmm_launch<V, 8>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 16) spmm_launch<V, 16>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+  else if (k <= 32) spmm_launch<V, 32>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+  else spmm_launch<V, 64>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+}
+
+}  // namespace
+
+void convert_dia(Matrix& A, const lespmv_params_t& p, hipStream_t s) {
+  LSP_REQUIRE(p.alignment > 0, LESPMV_ERR_ARG, "alignment must be positive");
+  auto d = std::make_unique<DiaDev>();
+  const int64_t len = A.m + A.n;
+  DBuf first, flag, pos;
+  first.alloc(sizeof(int) * std::max<int64_t>(len, 1));
+  flag.alloc(sizeof(int) * std::max<int64_t>(len, 1));
+  pos.alloc(sizeof(int64_t) * (len + 1));
+  {  // INT32_MAX marks an unoccupied diagonal
+    std::vector<int> init(std::max<int64_t>(len, 1), INT32_MAX);
+    LSP_HIP(hipMemcpyAsync(first.p, init.data(), sizeof(int) * init.size(), hipMemcpyHostToDevice, s));
+    LSP_HIP(hipStreamSynchronize(s));
+  }
+  if (A.m) {
+    hipLaunchKernelGGL(diag_first, grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, A.rp.as<int>(), A.ci.as<int>(),
+                       first.as<int>());
+    LSP_LAUNCHED();
+  }
+  hipLaunchKernelGGL(diag_flags, grid1d(len, kBlock), dim3(kBlock), 0, s, len, first.as<int>(), flag.as<int>());
+  LSP_LAUNCHED();
+  const int64_t nd = scan_exclusive_i32_to_i64(flag.as<int32_t>(), pos.as<int64_t>(), len, s);
+  d->ndiag = (int32_t)nd;
+  if (nd > p.max_diags) {
+    throw Error(LESPMV_ERR_MAX_DIAGS,
+                "Number of diagonals (" + std::to_string(nd) + ") excedes limit (" + std::to_string(p.max_diags) + ")");
+  }
+  DBuf keys, idx, rank;
+  keys.alloc(sizeof(uint32_t) * std::max<int64_t>(nd, 1));
+  idx.alloc(sizeof(uint32_t) * std::max<int64_t>(nd, 1));
+  rank.alloc(sizeof(int) * std::max<int64_t>(len, 1));
+  hipLaunchKernelGGL(diag_compact, grid1d(len, kBlock), dim3(kBlock), 0, s, len, first.as<int>(), pos.as<int64_t>(),
+                     keys.as<uint32_t>(), idx.as<uint32_t>());
+  LSP_LAUNCHED();
+  radix_sort_pairs(keys.as<uint32_t>(), idx.as<uint32_t>(), nd, bits_for((uint64_t)std::max<int64_t>(A.nnz, 1)), s);
+  d->offsets.alloc(sizeof(int64_t) * std::max<int64_t>(nd, 1));
+  if (nd) {
+    hipLaunchKernelGGL(diag_rank, grid1d(nd, kBlock), dim3(kBlock), 0, s, nd, A.m, idx.as<uint32_t>(),
+                       rank.as<int>(), d->offsets.as<int64_t>());
+    LSP_LAUNCHED();
+  }
+  d->stride = align_up(A.m, p.alignment);
+  d->data.alloc(vsize(A.dtype) * nd * d->stride);
+  LSP_HIP(hipMemsetAsync(d->data.p, 0, vsize(A.dtype) * nd * d->stride, s));
+  if (A.m) {
+    if (A.dtype == LESPMV_F64)
+      hipLaunchKernelGGL((dia_fill<double>), grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, A.rp.as<int>(),
+                         A.ci.as<int>(), A.val.as<double>(), rank.as<int>(), d->stride, d->data.as<double>());
+    else
+      hipLaunchKernelGGL((dia_fill<float>), grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, A.rp.as<int>(),
+                         A.ci.as<int>(), A.val.as<float>(), rank.as<int>(), d->stride, d->data.as<float>());
+    LSP_LAUNCHED();
+  }
+  auto offs = to_host<int64_t>(d->offsets.p, nd, s);
+  if (nd) {
+    d->min_off = *std::min_element(offs.begin(), offs.end());
+    d->max_off = *std::max_element(offs.begin(), offs.end());
+  }
+  plan_windows(*d, offs, vsize(A.dtype), s);
+  A.fmt[LESPMV_DIA] = std::move(d);
+}
+
+void dia_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  auto& d = *static_cast<DiaDev*>(A.fmt[LESPMV_DIA].get());
+  if (A.m == 0) return;
+  if (A.dtype == LESPMV_F64) run<double>(A, d, variant, alpha, x, beta, y, s);
+  else run<float>(A, d, variant, alpha, x, beta, y, s);
+}
+
+void dia_spmm(Matrix& A, int variant, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
+              int64_t ldy, hipStream_t s) {
+  (void)variant;
+  auto& d = *static_cast<DiaDev*>(A.fmt[LESPMV_DIA].get());
+  if (A.m == 0 || k == 0) return;
+  if (A.dtype == LESPMV_F64) spmm_dispatch<double>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+  else spmm_dispatch<float>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+}
+
+}  // namespace lsp

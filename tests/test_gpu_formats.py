@@ -1,0 +1,176 @@
+"""Every non-CSR format on the GPU: conversion parity and SpMV/SpMM parity.
+
+Conversions run on the GPU and are read back in the reference layout; every
+index and value array must equal the oracle's (itself pinned to the compiled
+reference by tests/test_oracle_golden.py) bit for bit, at the reference's
+parameters and at GPU-native ones. y must match the FP64 CSR oracle to 1e-12
+(FP64) / 1e-5 (FP32) relative 2-norm.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import BSR, COO, CSR5, DIA, ELL, SELL, SELL_C_R, SELL_C_SIGMA
+from tests.conftest import TOL, rel2
+from tests.fmt_helpers import compare_exports, lib_params, param_sets
+from tests.matrices import HostCsr, suite
+
+pytestmark = pytest.mark.gpu
+
+FORMATS = [ELL, SELL, SELL_C_SIGMA, SELL_C_R, COO, DIA, BSR, CSR5]
+
+
+def oracle_y(oracle, A, alpha, x, beta, y0):
+    y = y0.astype(np.float64).copy()
+    oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v.astype(np.float64), alpha, x.astype(np.float64), beta, y)
+    return y
+
+
+def available(gpu, fmt):
+    A = suite()["lap8x8"]
+    M = gpu.Matrix.from_host(A)
+    try:
+        M.convert(fmt, None)
+        return True
+    except gpu.LespmvError as e:
+        if e.code == gpu.ERR_UNSUPPORTED:
+            return False
+        raise
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_format_conversion_and_spmv(gpu, oracle, fmt, dt):
+    if not available(gpu, fmt):
+        pytest.skip(f"format {fmt} not built in this library")
+    rng = np.random.default_rng(fmt * 10 + (dt == np.float32))
+    worst = 0.0
+    for name, A in suite(dt).items():
+        M = gpu.Matrix.from_host(A)
+        x = rng.uniform(-1, 1, A.n).astype(dt)
+        y0 = rng.uniform(-1, 1, A.m).astype(dt)
+        ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+        ref0 = oracle_y(oracle, A, 1.0, x, 0.0, y0)
+        for prm in param_sets(fmt, dt):
+            ctx = (name, np.dtype(dt).name, fmt, list(prm))
+            rc, res = oracle.convert(A.m, A.n, A.rp, A.ci, A.v, fmt, prm)
+            try:
+                M.convert(fmt, lib_params(gpu, fmt, prm))
+                grc = 0
+            except gpu.LespmvError as e:
+                grc = e.code
+            assert grc == rc, (ctx, grc, rc, res.error())
+            if rc:
+                continue
+            mask = None
+            if fmt == CSR5:
+                mask = np.ones(res.scalar("num_offsets"), bool)  # every slot is defined by the oracle
+            compare_exports(M, res, fmt, dt, ctx, mask)
+            dx = gpu.DeviceArray.from_numpy(x)
+            for variant in (0, 1, 2) if fmt in (DIA, COO) else (0,):
+                dy = gpu.DeviceArray.from_numpy(y0)
+                M.spmv(fmt, 0.8, dx, 0.7, dy, variant=variant)
+                err = rel2(dy.numpy(), ref)
+                worst = max(worst, err)
+                assert err <= TOL[np.dtype(dt)], (ctx, variant, err)
+                dy = gpu.DeviceArray.from_numpy(np.full(A.m, np.nan, dt))
+                M.spmv(fmt, 1.0, dx, 0.0, dy, variant=variant)
+                y = dy.numpy()
+                assert np.all(np.isfinite(y)), (ctx, variant)
+                assert rel2(y, ref0) <= TOL[np.dtype(dt)], (ctx, variant)
+            M.release(fmt)
+    print(f"format {fmt} {np.dtype(dt).name}: worst rel2 vs FP64 oracle {worst:.3e}")
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_format_matches_reference_fixture(gpu, golden, fmt):
+    """GPU conversion vs the compiled reference's arrays stored in the fixture."""
+    if not available(gpu, fmt):
+        pytest.skip(f"format {fmt} not built in this library")
+    from oracle.oracle import SCALARS, array_dtypes
+    n = 0
+    for k in golden.files:
+        p = k.split("/")
+        if len(p) != 5 or p[4] != "params" or int(p[2]) != fmt:
+            continue
+        dn, name, _, pi = p[:4]
+        base = f"{dn}/{name}/{fmt}/{pi}"
+        if int(golden[f"{base}/rc"]) != 0:
+            continue
+        g = lambda s: golden[f"{dn}/{name}/{s}"]
+        A = HostCsr(int(g("m")), int(g("n")), g("rp"), g("ci"), g("v"))
+        M = gpu.Matrix.from_host(A)
+        M.convert(fmt, lib_params(gpu, fmt, golden[f"{base}/params"]))
+        for arr, adt in array_dtypes(fmt, A.v.dtype).items():
+            if f"{base}/{arr}" not in golden.files:
+                continue
+            got, exp = M.export(fmt, arr, adt), golden[f"{base}/{arr}"]
+            if arr == "tile_desc_offset":
+                w = golden[f"{base}/tile_desc_offset_written"]
+                got, exp = got[w], exp[w]
+            assert got.shape == exp.shape and np.array_equal(got, exp), (base, arr)
+            n += 1
+        for s in SCALARS[fmt]:
+            assert M.export_scalar(fmt, s) == int(golden[f"{base}/{s}"]), (base, s)
+        if f"{base}/y" in golden.files:
+            dx = gpu.DeviceArray.from_numpy(g("x"))
+            dy = gpu.DeviceArray.from_numpy(np.zeros(A.m, A.v.dtype))
+            M.spmv(fmt, 0.8, dx, 0.7, dy)
+            assert rel2(dy.numpy(), golden[f"{base}/y"]) <= TOL[A.v.dtype], base
+    assert n > 0
+
+
+def test_sell_sigma_must_divide(gpu):
+    A = suite()["rand200"]
+    M = gpu.Matrix.from_host(A)
+    with pytest.raises(gpu.LespmvError) as e:
+        M.convert(SELL_C_SIGMA, gpu.make_params(chunk=4, sigma=6, alignment=8))
+    assert e.value.code == gpu.ERR_UNSUPPORTED
+
+
+def test_not_converted_is_an_error(gpu):
+    A = suite()["rand200"]
+    M = gpu.Matrix.from_host(A)
+    dx = gpu.DeviceArray.from_numpy(np.ones(A.n))
+    dy = gpu.DeviceArray.from_numpy(np.ones(A.m))
+    with pytest.raises(gpu.LespmvError) as e:
+        M.spmv(SELL_C_SIGMA, 1.0, dx, 0.0, dy)
+    assert e.value.code == gpu.ERR_NOT_CONVERTED
+
+
+@pytest.mark.parametrize("fmt", [ELL, SELL, SELL_C_SIGMA, SELL_C_R, COO, DIA, BSR])
+@pytest.mark.parametrize("k", [1, 4, 8, 16, 32])
+def test_format_spmm_columns_equal_spmv(gpu, oracle, fmt, k):
+    if not available(gpu, fmt):
+        pytest.skip(f"format {fmt} not built in this library")
+    A = suite()["banded"] if fmt == DIA else suite()["rand_long"]
+    M = gpu.Matrix.from_host(A)
+    M.convert(fmt, None)
+    rng = np.random.default_rng(k + 100 * fmt)
+    X = rng.uniform(-1, 1, (A.n, k))
+    Y0 = rng.uniform(-1, 1, (A.m, k))
+    dX, dY = gpu.DeviceArray.from_numpy(X), gpu.DeviceArray.from_numpy(Y0)
+    M.spmm(fmt, k, 0.8, dX, k, 0.7, dY, k)
+    Y = dY.numpy()
+    for j in range(k):
+        ref = oracle_y(oracle, A, 0.8, X[:, j].copy(), 0.7, Y0[:, j].copy())
+        assert rel2(Y[:, j], ref) <= 1e-12, (fmt, k, j)
+
+
+@pytest.mark.parametrize("fmt", [SELL_C_SIGMA, SELL_C_R])
+def test_generated_lognormal_sliced(gpu, oracle, fmt):
+    """A 200k-row lognormal matrix (config-2 family): GPU sigma sort vs oracle,
+    at the reference parameters and at C=64."""
+    A = gpu.generate(gpu.GEN_LOGNORMAL, [200000, 200000, 16.0, 1.0, 4096, 0.3, 64], 2)
+    M = gpu.Matrix.from_host(A)
+    x = gpu.fill_uniform(A.n, 4)
+    ref = oracle_y(oracle, A, 1.0, x, 0.0, np.zeros(A.m))
+    for prm in ([4, 16384, 8, 0], [64, 4096, 1, 0]):
+        rc, res = oracle.convert(A.m, A.n, A.rp, A.ci, A.v, fmt, prm)
+        M.convert(fmt, lib_params(gpu, fmt, prm))
+        assert np.array_equal(M.export(fmt, "reorder", np.int32), res.get("reorder", np.int32))
+        assert np.array_equal(M.export(fmt, "chunk_len", np.int32), res.get("chunk_len", np.int32))
+        dx = gpu.DeviceArray.from_numpy(x)
+        dy = gpu.DeviceArray.empty(A.m, np.float64)
+        M.spmv(fmt, 1.0, dx, 0.0, dy)
+        assert rel2(dy.numpy(), ref) <= 1e-12
+        M.release(fmt)
