@@ -1,17 +1,20 @@
 #!/usr/bin/env python3
-"""Single-GPU SpMV benchmark on the lognormal 4,194,304 x 4,194,304 matrix.
+"""Single-GPU SpMV benchmark: the all-formats single-matrix workload.
 
-Workload: per-row nnz lognormal (mean 16, sigma 1, capped at 4096), 70 % of
-columns uniform over [0, N) and 30 % within +-64 of the diagonal, values
-U[-1,1), FP64, int32 indices; x = U[-1,1) (seed 4). One step = one
-y = A*x on the GPU with the matrix and vectors resident in HBM.
+Workload (one matrix per section, FP64 unless stated, int32 indices):
+  * lognormal: M = N = 4,194,304, per-row nnz lognormal (mean 16, sigma 1,
+    capped at 4096), 70 % of columns uniform over [0, N) and 30 % within +-64 of
+    the diagonal, values U[-1,1), x = U[-1,1) (seed 4); FP64 (headline) and FP32;
+  * stencil27: 27-point stencil on a 160^3 grid (diag 26, off-diagonals -1) for
+    the DIA and BSR paths.
+One step = one y = A*x with the matrix and vectors resident in HBM.
 
 Prints ONE JSON line. `value` is the effective bandwidth (algorithmic bytes:
-values + indices + x read once + y written) of the best format measured,
-divided by the wall time per step. `roofline` compares the same bytes with the
-kernel's device time (hipEvents on its stream) against 8 TB/s. `cpu_baseline`
-times the CPU oracle's kernel_flag=2 CSR SpMV (the reference's load-balanced
-OpenMP path, restated) on this host.
+values + indices + x read once + y written) of the fastest kernel (least time)
+on the FP64 lognormal matrix, divided by the wall time per step. `roofline` compares the
+same bytes with that kernel's device time (hipEvents on its stream) against
+MI355X's 8 TB/s. `cpu_baseline` times the CPU oracle's kernel_flag=2 CSR SpMV
+(the reference's load-balanced OpenMP path, restated) on this host's cores.
 """
 from __future__ import annotations
 
@@ -27,27 +30,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-WORKLOAD = dict(m=4194304, n=4194304, mean=16.0, sigma=1.0, cap=4096, frac_local=0.3, band=64, seed=2, x_seed=4)
+LOGNORMAL = [4194304, 4194304, 16.0, 1.0, 4096, 0.3, 64]
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def candidate_formats(lib):
-    """(label, fmt, variant, params) of every kernel measured on the workload."""
-    c = [("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None), ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None)]
-    extra = [
-        ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, lib.make_params(chunk=64, sigma=4096, alignment=1)),
-        ("SELL-64-R", lib.SELL_C_R, 0, lib.make_params(chunk=64, alignment=1)),
-        ("CSR5-64x16", lib.CSR5, 0, lib.make_params(omega=64, sigma=16)),
-        ("COO", lib.COO, 0, None),
-    ]
-    return c + extra
+def kernels(lib, section):
+    """(label, fmt, variant, params) measured per section."""
+    sell = lambda C, s: lib.make_params(chunk=C, sigma=s, alignment=1)
+    if section == "stencil27":
+        return [("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None), ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0,
+                                                                    sell(64, 4096)),
+                ("DIA", lib.DIA, lib.VARIANT_VECTOR, lib.make_params(alignment=1, max_diags=10240)),
+                ("DIA-lds", lib.DIA, lib.VARIANT_MERGE, lib.make_params(alignment=1, max_diags=10240)),
+                ("BSR-4x4", lib.BSR, 0, lib.make_params(block_rows=4, block_cols=4))]
+    ks = [("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None), ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None),
+          ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, sell(64, 4096)), ("SELL-64-R", lib.SELL_C_R, 0, sell(64, 0)),
+          ("COO", lib.COO, 0, None), ("CSR5-64x16", lib.CSR5, 0, lib.make_params(omega=64, sigma=16))]
+    return ks if section == "lognormal" else ks[:3]
 
 
 def traffic_from_profiles(label):
-    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
+    """HBM bytes per launch from rocprofv3 --pmc (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -58,19 +64,55 @@ def traffic_from_profiles(label):
         return None
 
 
+def measure(lib, M, dx, dy, section, wanted, warmup):
+    results = []
+    for label, fmt, variant, params in kernels(lib, section):
+        if wanted and label not in wanted:
+            continue
+        try:
+            if fmt != lib.CSR:
+                M.convert(fmt, params)
+            conv_ms = M.convert_ms(fmt) if fmt != lib.CSR else 0.0
+        except lib.LespmvError as e:
+            log(f"[{section}] {label}: skipped ({e})")
+            continue
+        eff = M.effective_bytes(fmt, 1, False)
+        M.time_spmv(fmt, 1.0, dx, 0.0, dy, max(warmup, 1), variant)
+        ms = M.time_spmv(fmt, 1.0, dx, 0.0, dy, 50, variant)
+        results.append(dict(label=label, fmt=fmt, variant=variant, params=params, ms=ms, gbs=eff / ms / 1e6,
+                            effective_bytes=eff, convert_ms=conv_ms, format_bytes=M.format_bytes(fmt)))
+        log(f"[{section}] {label}: {ms * 1e3:.1f} us  {eff / ms / 1e6:.0f} GB/s  convert {conv_ms:.1f} ms")
+        if fmt != lib.CSR:
+            M.release(fmt)
+    return results
+
+
+def summary(results):
+    return {r["label"]: {"us": round(r["ms"] * 1e3, 2), "GBs": round(r["gbs"], 1), "frac": round(r["gbs"] * 1e9 /
+                                                                                                  HBM_PEAK, 4),
+                         "convert_ms": round(r["convert_ms"], 2), "format_bytes": r["format_bytes"]}
+            for r in results}
+
+
 def cpu_baseline(A, x, threads, max_seconds):
     from oracle.oracle import Oracle
     O = Oracle()
     y = np.zeros(A.m)
-    # reference iteration policy (include/spmv_benchmark.h:185-195) bounded by max_seconds
+    # reference iteration policy (include/spmv_benchmark.h:185-195), bounded by max_seconds
     t_one = O.time_csr_lb(A.m, A.n, A.rp, A.ci, A.v, x, y, threads, 1)
-    if t_one < 20.0:
-        iters = 1000
-    else:
-        iters = int(min(1000, max(20, 20000.0 / t_one)))
+    iters = 1000 if t_one < 20.0 else int(min(1000, max(20, 20000.0 / t_one)))
     iters = max(1, min(iters, int(max_seconds * 1000.0 / max(t_one, 1e-3))))
     ms = O.time_csr_lb(A.m, A.n, A.rp, A.ci, A.v, x, y, threads, iters)
     return ms, iters, y
+
+
+def copy_bandwidth(lib, nbytes=1 << 30, iters=20):
+    src = lib.DeviceArray(nbytes)
+    dst = lib.DeviceArray(nbytes)
+    src.fill_bytes(1)
+    lib.time_copy(dst, src, nbytes, 2)
+    ms = lib.time_copy(dst, src, nbytes, iters)
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9, ms
 
 
 def main():
@@ -80,55 +122,39 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bound on the CPU baseline's timed work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--formats", default="all", help="comma list of labels to measure, or 'all'")
-    ap.add_argument("--scale", type=float, default=1.0, help="shrink M=N for quick checks (not for reporting)")
+    ap.add_argument("--formats", default="all", help="comma list of kernel labels to measure, or 'all'")
+    ap.add_argument("--sections", default="lognormal,lognormal_fp32,stencil27,copy")
+    ap.add_argument("--copy-only", action="store_true", help="only run the plain copy kernel (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the single-matrix SpMV workload "
-              "is not sharded across GPUs in this build.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build runs the single-matrix "
+              "SpMV workload on one MI355X and does not shard it.", file=sys.stderr)
         sys.exit(2)
 
     from sspmv_amd import lib
     lib.check(lib.load().lespmv_init(0))
-    wl = dict(WORKLOAD)
-    if args.scale != 1.0:
-        wl["m"] = wl["n"] = int(wl["m"] * args.scale)
+    if args.copy_only:
+        gbs, ms = copy_bandwidth(lib, iters=5)
+        print(json.dumps({"copy_GBs": round(gbs, 1), "copy_bytes_per_launch": 2 * (1 << 30), "copy_ms": ms}))
+        return
+    sections = set(args.sections.split(","))
+    wanted = None if args.formats == "all" else set(args.formats.split(","))
+
     t0 = time.time()
-    A = lib.generate(lib.GEN_LOGNORMAL, [wl["m"], wl["n"], wl["mean"], wl["sigma"], wl["cap"], wl["frac_local"],
-                                         wl["band"]], wl["seed"], np.float64)
-    x = lib.fill_uniform(A.n, wl["x_seed"], np.float64)
-    log(f"generated M={A.m} nnz={A.nnz} in {time.time() - t0:.1f}s")
+    A = lib.generate(lib.GEN_LOGNORMAL, LOGNORMAL, 2, np.float64)
+    x = lib.fill_uniform(A.n, 4, np.float64)
+    log(f"generated lognormal M={A.m} nnz={A.nnz} in {time.time() - t0:.1f}s")
     M = lib.Matrix.from_host(A)
     dx = lib.DeviceArray.from_numpy(x)
     dy = lib.DeviceArray.empty(A.m, np.float64)
-
-    results = []
-    wanted = None if args.formats == "all" else set(args.formats.split(","))
-    for label, fmt, variant, params in candidate_formats(lib):
-        if wanted and label not in wanted:
-            continue
-        try:
-            if fmt != lib.CSR or not M.has(fmt):
-                M.convert(fmt, params)
-            conv_ms = M.convert_ms(fmt) if fmt != lib.CSR else 0.0
-        except lib.LespmvError as e:
-            log(f"{label}: skipped ({e})")
-            continue
-        eff = M.effective_bytes(fmt, 1, False)
-        M.time_spmv(fmt, 1.0, dx, 0.0, dy, max(args.warmup, 1), variant)  # warm-up
-        ms = M.time_spmv(fmt, 1.0, dx, 0.0, dy, 50, variant)
-        results.append(dict(label=label, fmt=fmt, variant=variant, ms=ms, gbs=eff / ms / 1e6, effective_bytes=eff,
-                            convert_ms=conv_ms, format_bytes=M.format_bytes(fmt)))
-        log(f"{label}: {ms * 1e3:.1f} us  {eff / ms / 1e6:.0f} GB/s  convert {conv_ms:.1f} ms")
-        if fmt not in (lib.CSR,):
-            M.release(fmt)
-    best = max(results, key=lambda r: r["gbs"])
+    results = measure(lib, M, dx, dy, "lognormal", wanted, args.warmup)
+    best = min(results, key=lambda r: r["ms"])  # fastest kernel on this matrix
     fmt, variant = best["fmt"], best["variant"]
     if fmt != lib.CSR:
-        M.convert(fmt, [c for c in candidate_formats(lib) if c[0] == best["label"]][0][3])
+        M.convert(fmt, best["params"])
 
-    # warm-up, then the timed region: K steps bracketed by device syncs; the
-    # kernel's own device time comes from hipEvents on its stream
+    # timed region: K steps bracketed by device syncs; the kernel's own device
+    # time comes from hipEvents recorded on the stream it runs on
     M.time_spmv(fmt, 1.0, dx, 0.0, dy, max(args.warmup, 1), variant)
     lib.synchronize()
     t0 = time.perf_counter()
@@ -139,7 +165,7 @@ def main():
     y_gpu = dy.numpy()
 
     out = {
-        "metric": "SpMV effective GB/s (values+indices+x+y) and fraction of HBM peak",
+        "metric": "SpMV effective GB/s (values+indices+x read once+y) and fraction of HBM peak",
         "value": round(eff / (wall_ms * 1e-3) / 1e9, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -152,10 +178,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"lognormal SpMV M=N={A.m} nnz={A.nnz} (row nnz lognormal mean 16 sigma 1 cap 4096; "
-                        "70% cols uniform, 30% within +-64 of diagonal), FP64 values, int32 indices",
-            "format": best["label"],
-            "m": A.m, "nnz": A.nnz,
+            "workload": f"single-matrix SpMV, lognormal M=N={A.m} nnz={A.nnz} (row nnz lognormal mean 16 sigma 1 "
+                        "cap 4096; 70% cols uniform, 30% within +-64 of the diagonal), FP64 values, int32 indices",
+            "format": best["label"], "m": A.m, "nnz": A.nnz,
         },
         "roofline": {
             "bound": "hbm",
@@ -167,24 +192,38 @@ def main():
             "algorithmic_bytes": eff,
             "kernel_ms": round(dev_ms, 5),
         },
-        "formats": {r["label"]: {"us": round(r["ms"] * 1e3, 2), "GBs": round(r["gbs"], 1),
-                                 "convert_ms": round(r["convert_ms"], 2), "bytes": r["format_bytes"]}
-                    for r in results},
+        "formats": {"lognormal_f64": summary(results)},
     }
+    if "copy" in sections:
+        gbs, _ = copy_bandwidth(lib)
+        out["copy_GBs"] = round(gbs, 1)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         ms, iters, y_cpu = cpu_baseline(A, x, threads, args.cpu_seconds)
-        eff_csr = M.effective_bytes(lib.CSR, 1, False)
         out["cpu_baseline"] = {
-            "value": round(eff_csr / (ms * 1e-3) / 1e9, 3),
+            "value": round(M.effective_bytes(lib.CSR) / (ms * 1e-3) / 1e9, 3),
             "unit": "GB/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"the full workload matrix, {iters} CSR SpMVs (kernel_flag=2 restated, OpenMP), "
+            "sample": f"the full lognormal matrix, {iters} CSR SpMVs (kernel_flag=2 restated, OpenMP), "
                       f"{ms:.2f} ms each",
         }
         nb = np.linalg.norm(y_cpu)
         out["parity_rel2_vs_cpu"] = float(np.linalg.norm(y_gpu - y_cpu) / (nb if nb else 1.0))
+    del M, dx, dy
+    if "lognormal_fp32" in sections:
+        A32 = lib.generate(lib.GEN_LOGNORMAL, LOGNORMAL, 2, np.float32)
+        M = lib.Matrix.from_host(A32)
+        dx = lib.DeviceArray.from_numpy(x.astype(np.float32))
+        dy = lib.DeviceArray.empty(A32.m, np.float32)
+        out["formats"]["lognormal_f32"] = summary(measure(lib, M, dx, dy, "lognormal_fp32", wanted, args.warmup))
+        del M, dx, dy, A32
+    if "stencil27" in sections:
+        S = lib.generate(lib.GEN_STENCIL27, [160, 26.0, -1.0], 0, np.float64)
+        M = lib.Matrix.from_host(S)
+        dx = lib.DeviceArray.from_numpy(lib.fill_uniform(S.n, 5))
+        dy = lib.DeviceArray.empty(S.m, np.float64)
+        out["formats"]["stencil27_f64"] = summary(measure(lib, M, dx, dy, "stencil27", wanted, args.warmup))
     print(json.dumps(out), flush=True)
 
 

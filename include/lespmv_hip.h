@@ -168,6 +168,9 @@ int lespmv_memcpy_h2d(void* dst, const void* src, int64_t bytes);
 int lespmv_memcpy_d2h(void* dst, const void* src, int64_t bytes);
 int lespmv_memset(void* dptr, int value, int64_t bytes);
 int lespmv_synchronize(void);
+/* Time `iters` copies of `bytes` (multiple of 16) with the library's plain
+ * streaming copy kernel (reference point for HBM counters). Synchronous. */
+int lespmv_time_copy(void* dst, const void* src, int64_t bytes, int32_t iters, double* ms_per_copy);
 int lespmv_stream_create(lespmv_stream_t* s);
 int lespmv_stream_destroy(lespmv_stream_t s);
 int lespmv_stream_synchronize(lespmv_stream_t s);

@@ -508,6 +508,25 @@ int lespmv_memset(void* dptr, int value, int64_t bytes) {
 int lespmv_synchronize(void) {
   return guarded([&] { LSP_HIP(hipDeviceSynchronize()); });
 }
+int lespmv_time_copy(void* dst, const void* src, int64_t bytes, int32_t iters, double* ms_per_copy) {
+  return guarded([&] {
+    LSP_REQUIRE(iters > 0 && ms_per_copy, LESPMV_ERR_ARG, "iters must be > 0");
+    hipStream_t s = nullptr;
+    hipEvent_t e0, e1;
+    LSP_HIP(hipEventCreate(&e0));
+    LSP_HIP(hipEventCreate(&e1));
+    LSP_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; i++) device_copy(dst, src, bytes, s);
+    LSP_HIP(hipEventRecord(e1, s));
+    LSP_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LSP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_copy = ms / iters;
+  });
+}
+
 int lespmv_stream_create(lespmv_stream_t* s) {
   return guarded([&] {
     hipStream_t st;

@@ -211,7 +211,25 @@ void radix_sort_impl(K* keys, uint32_t* vals, int64_t n, int bits, hipStream_t s
   LSP_HIP(hipStreamSynchronize(s));
 }
 
+// Plain streaming copy (16-byte vectors): the reference point for HBM byte
+// counters and for the achievable copy bandwidth.
+__global__ __launch_bounds__(kBlock) void copy_kernel(const int4* __restrict__ src, int4* __restrict__ dst,
+                                                      int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+void device_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
+  LSP_REQUIRE(bytes % 16 == 0, LESPMV_ERR_ARG, "copy size must be a multiple of 16 bytes");
+  const int64_t n16 = bytes / 16;
+  if (n16 == 0) return;
+  const int64_t blocks = std::min<int64_t>(ceil_div(n16, kBlock), 256 * 64);
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, static_cast<const int4*>(src),
+                     static_cast<int4*>(dst), n16);
+  LSP_LAUNCHED();
+}
 
 int64_t reduce_max_rowlen(const int32_t* rp, int64_t m, hipStream_t s) {
   DBuf out;

@@ -87,6 +87,7 @@ def load() -> C.CDLL:
         "lespmv_memcpy_d2h": ([vp, vp, i64], C.c_int),
         "lespmv_memset": ([vp, C.c_int, i64], C.c_int),
         "lespmv_synchronize": ([], C.c_int),
+        "lespmv_time_copy": ([vp, vp, i64, i32, C.POINTER(dbl)], C.c_int),
         "lespmv_stream_create": ([C.POINTER(vp)], C.c_int),
         "lespmv_stream_destroy": ([vp], C.c_int),
         "lespmv_stream_synchronize": ([vp], C.c_int),
@@ -304,6 +305,12 @@ class Matrix:
 
     def export_scalar(self, fmt: int, name: str) -> int:
         return int(self.export(fmt, name, np.int64)[0])
+
+
+def time_copy(dst: DeviceArray, src: DeviceArray, nbytes: int, iters: int) -> float:
+    ms = C.c_double()
+    check(load().lespmv_time_copy(dst.ptr, src.ptr, nbytes, iters, C.byref(ms)))
+    return ms.value
 
 
 def synchronize():
