@@ -175,6 +175,16 @@ int lespmv_stream_create(lespmv_stream_t* s);
 int lespmv_stream_destroy(lespmv_stream_t s);
 int lespmv_stream_synchronize(lespmv_stream_t s);
 
+/* MatrixMarket input, replacing read_csr_matrix (utils/sparse_io.cpp:179-204):
+ * coordinate real/integer/pattern, symmetric expansion, then coo_to_csr with
+ * optional duplicate compaction. Host arrays owned by the library. */
+int lespmv_read_mtx(const char* path, int dtype, int compact, lespmv_host_csr_t* out);
+/* coo_to_csr (include/sparse_conversion.h:29-91): stable by row; compact = 0
+ * runs as a GPU radix sort, compact = 1 sums duplicates like
+ * sum_csr_duplicates (include/sparse_operation.h:21-73). Host input/output. */
+int lespmv_coo_to_csr(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* row_index,
+                      const int32_t* col_index, const void* values, int compact, lespmv_host_csr_t* out);
+
 /* Synthetic inputs (host). */
 int lespmv_generate(int family, const double* params, int32_t nparams, uint64_t seed, int dtype,
                     lespmv_host_csr_t* out);

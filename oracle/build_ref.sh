@@ -20,6 +20,7 @@ SRCS=(
   "$REF/src/spmv_csr.cpp" "$REF/src/spmv_coo.cpp" "$REF/src/spmv_ell.cpp" "$REF/src/spmv_s_ell.cpp"
   "$REF/src/spmv_sell_c_sigma.cpp" "$REF/src/spmv_sell_c_R.cpp" "$REF/src/spmv_dia.cpp" "$REF/src/spmv_bsr.cpp"
   "$REF/src/spmv_csr5.cpp" "$REF/utils/thread.cpp" "$REF/utils/partition.cpp"
+  "$REF/utils/sparse_io.cpp" "$REF/utils/mmio.cpp"
 )
 "$CXX" -O2 -std=c++17 -fPIC -shared -fopenmp -march=native -w \
   -I"$REF/include" -I"$HERE" \

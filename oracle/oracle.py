@@ -207,6 +207,7 @@ class Reference:
         lib.ref_coo_to_csr.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p, C.c_void_p,
                                        C.c_int]
         lib.ref_set_threads.argtypes = [C.c_int]
+        lib.ref_read_mtx.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
         self.lib = lib
 
     def set_threads(self, t: int):
@@ -217,6 +218,11 @@ class Reference:
         rc = self.lib.ref_run(r.h, _dtype_id(v), fmt, m, n, len(ci), rp, ci, _ptr(v),
                               np.asarray(params, dtype=np.int64), flag, alpha, _ptr(x), beta, _ptr(y))
         return rc, r
+
+    def read_mtx(self, path: str, dtype=np.float64, compact=False) -> Result:
+        r = Result(self.lib, "ref")
+        self.lib.ref_read_mtx(r.h, 1 if np.dtype(dtype) == np.float64 else 0, path.encode(), int(compact))
+        return r
 
     def coo_to_csr(self, m, n, ri, ci, v, compact=False) -> Result:
         r = Result(self.lib, "ref")

@@ -248,4 +248,22 @@ int ref_coo_to_csr(void* res, int dtype, int m, int n, int nnz, const int* ri, c
   return 0;
 }
 
+// read_csr_matrix (utils/sparse_io.cpp:179-204) on a MatrixMarket file.
+int ref_read_mtx(void* res, int dtype, const char* path, int compact) {
+  auto& r = *static_cast<NamedArrays*>(res);
+  auto go = [&](auto tag) {
+    using V = decltype(tag);
+    auto csr = read_csr_matrix<int, V>(path, compact != 0);
+    r.scalar("num_rows", csr.num_rows);
+    r.scalar("num_cols", csr.num_cols);
+    r.scalar("num_nnzs", csr.num_nnzs);
+    r.put("row_offset", csr.row_offset, csr.num_rows + 1);
+    r.put("col_index", csr.col_index, csr.num_nnzs);
+    r.put("values", csr.values, csr.num_nnzs);
+    delete_csr_matrix(csr);
+  };
+  if (dtype == 1) go(double{}); else go(float{});
+  return 0;
+}
+
 }  // extern "C"

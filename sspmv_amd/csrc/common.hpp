@@ -203,6 +203,8 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, hipSt
 void radix_sort_u64(uint64_t* keys, int64_t n, int bits, hipStream_t s);
 int64_t reduce_max_rowlen(const int32_t* rp, int64_t m, hipStream_t s);
 void device_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
+void coo_to_csr_device(int dtype, int64_t m, int64_t nnz, const int32_t* ri, const int32_t* ci, const void* v,
+                       DBuf& rp_out, DBuf& ci_out, DBuf& v_out, hipStream_t s);
 
 // Host-side copies.
 template <class T>

@@ -94,6 +94,8 @@ def load() -> C.CDLL:
         "lespmv_generate": ([C.c_int, C.POINTER(dbl), i32, C.c_uint64, C.c_int, C.POINTER(HostCsrC)], C.c_int),
         "lespmv_host_csr_free": ([C.POINTER(HostCsrC)], C.c_int),
         "lespmv_fill_uniform": ([vp, C.c_int, i64, C.c_uint64, dbl, dbl], C.c_int),
+        "lespmv_read_mtx": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(HostCsrC)], C.c_int),
+        "lespmv_coo_to_csr": ([C.c_int, i64, i64, i64, vp, vp, vp, C.c_int, C.POINTER(HostCsrC)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -195,12 +197,8 @@ class HostCsr:
         return int(self.ci.shape[0])
 
 
-def generate(family: int, params, seed: int, dtype=np.float64) -> HostCsr:
-    """Synthetic CSR from the library's counter-based generators (host)."""
+def _take_host_csr(out: HostCsrC) -> HostCsr:
     lib = load()
-    arr = (C.c_double * max(len(params), 1))(*params)
-    out = HostCsrC()
-    check(lib.lespmv_generate(family, arr, len(params), seed, dtype_id(dtype), C.byref(out)))
     try:
         m, nnz = out.m, out.nnz
         rp = np.ctypeslib.as_array(out.row_offset, shape=(m + 1,)).copy()
@@ -210,6 +208,33 @@ def generate(family: int, params, seed: int, dtype=np.float64) -> HostCsr:
     finally:
         lib.lespmv_host_csr_free(C.byref(out))
     return HostCsr(int(out.m), int(out.n), rp, ci, v)
+
+
+def generate(family: int, params, seed: int, dtype=np.float64) -> HostCsr:
+    """Synthetic CSR from the library's counter-based generators (host)."""
+    arr = (C.c_double * max(len(params), 1))(*params)
+    out = HostCsrC()
+    check(load().lespmv_generate(family, arr, len(params), seed, dtype_id(dtype), C.byref(out)))
+    return _take_host_csr(out)
+
+
+def read_mtx(path: str, dtype=np.float64, compact: bool = False) -> HostCsr:
+    """read_csr_matrix semantics (symmetric expansion, pattern -> 1.0)."""
+    out = HostCsrC()
+    check(load().lespmv_read_mtx(path.encode(), dtype_id(dtype), int(compact), C.byref(out)))
+    return _take_host_csr(out)
+
+
+def coo_to_csr(m: int, n: int, ri, ci, v, compact: bool = False) -> HostCsr:
+    """coo_to_csr; without compaction it runs as a GPU radix sort."""
+    ri = np.ascontiguousarray(ri, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    v = np.ascontiguousarray(v)
+    out = HostCsrC()
+    check(load().lespmv_coo_to_csr(dtype_id(v.dtype), m, n, len(ci), ri.ctypes.data_as(C.c_void_p),
+                                   ci.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), int(compact),
+                                   C.byref(out)))
+    return _take_host_csr(out)
 
 
 def fill_uniform(n: int, seed: int, dtype=np.float64, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:

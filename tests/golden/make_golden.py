@@ -53,6 +53,36 @@ def param_sets(fmt, dt):
     return ps
 
 
+MTX_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mtx")
+
+
+def write_mtx_fixtures():
+    """Small MatrixMarket files covering the reader's branches (written here)."""
+    os.makedirs(MTX_DIR, exist_ok=True)
+    r = np.random.default_rng(99)
+    files = {}
+    # general real with duplicates, comments and an unsorted entry order
+    ent = [(int(r.integers(1, 13)), int(r.integers(1, 10)), float(np.round(r.uniform(-3, 3), 3))) for _ in range(40)]
+    ent += [(5, 5, 1.5), (5, 5, -1.5), (7, 2, 0.25), (7, 2, 0.5)]
+    files["general_real.mtx"] = ("%%MatrixMarket matrix coordinate real general\n% a comment\n% another\n"
+                                 f"12 9 {len(ent)}\n" + "".join(f"{a} {b} {c}\n" for a, b, c in ent))
+    # symmetric real: lower triangle with diagonal
+    ent = [(i, j, float(np.round(r.uniform(-2, 2), 3))) for i in range(1, 11) for j in range(1, i + 1)
+           if r.random() < 0.4 or i == j]
+    files["symmetric_real.mtx"] = ("%%MatrixMarket matrix coordinate real symmetric\n"
+                                   f"10 10 {len(ent)}\n" + "".join(f"{a} {b} {c}\n" for a, b, c in ent))
+    ent = [(int(r.integers(1, 8)), int(r.integers(1, 16))) for _ in range(25)]
+    files["pattern_general.mtx"] = ("%%MatrixMarket matrix coordinate pattern general\n"
+                                    f"7 15 {len(ent)}\n" + "".join(f"{a} {b}\n" for a, b in ent))
+    ent = [(i, j, int(r.integers(-5, 6))) for i in range(1, 9) for j in range(1, i + 1) if r.random() < 0.5]
+    files["integer_symmetric.mtx"] = ("%%MatrixMarket matrix coordinate integer symmetric\n"
+                                      f"8 8 {len(ent)}\n" + "".join(f"{a} {b} {c}\n" for a, b, c in ent))
+    for name, text in files.items():
+        with open(os.path.join(MTX_DIR, name), "w") as f:
+            f.write(text)
+    return sorted(files)
+
+
 def key(*parts):
     return "/".join(str(p) for p in parts)
 
@@ -131,6 +161,15 @@ def main():
         res = ref.coo_to_csr(m, n, ri, ci, v, bool(compact))
         for arr, adt in (("row_offset", np.int32), ("col_index", np.int32), ("values", np.float64)):
             out[key("coo2csr", compact, arr)] = res.get(arr, adt)
+    # MatrixMarket inputs through read_csr_matrix (utils/sparse_io.cpp:26-204)
+    for name in write_mtx_fixtures():
+        path = os.path.join(MTX_DIR, name)
+        for compact in (0, 1):
+            res = ref.read_mtx(path, np.float64, bool(compact))
+            for arr, adt in (("row_offset", np.int32), ("col_index", np.int32), ("values", np.float64)):
+                out[key("mtx", name, compact, arr)] = res.get(arr, adt)
+            for sc in ("num_rows", "num_cols", "num_nnzs"):
+                out[key("mtx", name, compact, sc)] = np.int64(res.scalar(sc))
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT) / 1e6:.2f} MB")
 
