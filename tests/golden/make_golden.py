@@ -116,9 +116,15 @@ def main():
             for fmt in (COO, DIA, ELL, SELL, SELL_C_SIGMA, SELL_C_R, CSR5, BSR):
                 if fmt == CSR5 and dt != np.float64:
                     continue  # the reference instantiates LeSpMV_csr5 for double only
+                # The reference reads x[-1] for SELL pads (src/spmv_s_ell.cpp:44-48) and x past N in
+                # the last BSR block column (src/spmv_bsr.cpp:39-43); zeros around x make those reads
+                # deterministic (they are multiplied by zero padding).
+                xpad = np.zeros(A.n + 2 + 64, dt)
+                xpad[1:A.n + 1] = x
+                xv = xpad[1:]
                 for pi, prm in enumerate(param_sets(fmt, dt)):
                     y = np.zeros(A.m, dt)
-                    rc, res = ref.run(A.m, A.n, A.rp, A.ci, A.v, fmt, prm, 0, 0.8, x, 0.7, y)
+                    rc, res = ref.run(A.m, A.n, A.rp, A.ci, A.v, fmt, prm, 0, 0.8, xv, 0.7, y)
                     base = key(dn, name, fmt, pi)
                     out[key(base, "params")] = np.asarray(prm, np.int64)
                     out[key(base, "rc")] = np.int64(rc)
@@ -146,7 +152,7 @@ def main():
                     out[key(base, "y")] = y
                     if fmt not in (CSR5,):
                         yb = y0.copy()
-                        ref.run(A.m, A.n, A.rp, A.ci, A.v, fmt, prm, 0, 0.8, x, 0.7, yb)
+                        ref.run(A.m, A.n, A.rp, A.ci, A.v, fmt, prm, 0, 0.8, xv, 0.7, yb)
                         out[key(base, "y_beta")] = yb
     # coo_to_csr with duplicates, compact and not (include/sparse_conversion.h:29-91)
     r2 = np.random.default_rng(7)
