@@ -30,6 +30,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iostream>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -767,3 +769,179 @@ double test_csr5_matrix_kernels(const CSR_Matrix<IndexType, ValueType>& csr_ref,
   delete_host_matrix(a);
   return ms;
 }
+
+// ---- feature extractor (include/sparse_features.h, src/sparse_features.cpp) -------
+//
+// MTX keeps the reference's method names and call order (test/Get_features.cpp:
+// MtxLoad, CalculateFeatures, CalculateTilesExtraFeatures, FeaturesPrint,
+// ExtraFeaturesPrint, FeaturesWrite). MtxLoad reads the file into a device CSR;
+// CalculateFeatures computes every feature on the GPU in one call
+// (lespmv_features), including the tile features and the row distance, so the
+// two later Calculate* calls have nothing left to do. Statistics are computed in
+// double whatever ValueType is. FeaturesWrite takes the output root as its
+// argument instead of the reference's fixed /data/lsl/MModel-Data.
+
+template <typename IndexType, typename ValueType>
+class MTX {
+ public:
+  explicit MTX(IndexType matID = 0) : matrixID_(matID) { std::cout << "== MTX Features Extraction ==\n" << std::endl; }
+  IndexType getRowNum() const { return (IndexType)f_.num_rows; }
+  IndexType getColNum() const { return (IndexType)f_.num_cols; }
+  IndexType getTileSize() const { return (IndexType)LESPMV_FEAT_TILES; }
+  std::string getMatName() const { return matrixName; }
+  const lespmv_features_t& features() const { return f_; }
+
+  bool MtxLoad(const char* mat_path) {
+    const std::string p(mat_path);
+    const size_t slash = p.find_last_of("/\\");
+    const size_t start = slash == std::string::npos ? 0 : slash + 1;
+    size_t dot = p.find_last_of('.');
+    if (dot == std::string::npos || dot < start) dot = p.size();
+    matrixName = p.substr(start, dot - start);
+    lespmv_host_csr_t h;
+    lespmv::check(lespmv_read_mtx(mat_path, lespmv::dtype_of<ValueType>(), 0, &h), "MtxLoad");
+    flags_ = (h.mm_flags & LESPMV_MM_SYMMETRIC) ? LESPMV_FEAT_SYMMETRIC_FILE : 0;
+    handle_ = std::make_shared<lespmv::Handle>();
+    const int rc = lespmv_create_csr(&handle_->h, h.dtype, h.m, h.n, h.nnz, h.row_offset, h.col_index, h.values, 0);
+    lespmv_host_csr_free(&h);
+    lespmv::check(rc, "MtxLoad");
+    return true;
+  }
+
+  bool CalculateFeatures() {
+    if (!handle_) throw lespmv::Error(LESPMV_ERR_ARG, "MtxLoad first");
+    const size_t T2 = (size_t)LESPMV_FEAT_TILES * LESPMV_FEAT_TILES, T = LESPMV_FEAT_TILES;
+    tile_nnz_.assign(T2, 0); tile_max_.assign(T2, 0); tile_ave_.assign(T2, 0.0); tile_std_.assign(T2, 0.0);
+    rb_nnz_.assign(T, 0); rb_max_.assign(T, 0); rb_ave_.assign(T, 0.0); rb_std_.assign(T, 0.0);
+    cb_nnz_.assign(T, 0); cb_max_.assign(T, 0); cb_ave_.assign(T, 0.0); cb_std_.assign(T, 0.0);
+    lespmv_feature_images_t img{};
+    img.tile_nnz = tile_nnz_.data(); img.tile_max = tile_max_.data(); img.tile_ave = tile_ave_.data();
+    img.tile_std = tile_std_.data();
+    img.rb_nnz = rb_nnz_.data(); img.rb_max = rb_max_.data(); img.rb_ave = rb_ave_.data(); img.rb_std = rb_std_.data();
+    img.cb_nnz = cb_nnz_.data(); img.cb_max = cb_max_.data(); img.cb_ave = cb_ave_.data(); img.cb_std = cb_std_.data();
+    lespmv::check(lespmv_features(handle_->h, flags_, &f_, &img), "CalculateFeatures");
+    return true;
+  }
+  bool CalculateTilesFeatures() { return true; }
+  bool CalculateTilesExtraFeatures(const char*) { return true; }
+
+  void FeaturesPrint() const {
+    std::cout << "[" << matrixID_ << "]  " << matrixName << "counting by " << 8 * sizeof(IndexType)
+              << "-bit Index, 64-bit statistics" << std::endl;
+    std::cout << "is_symmetric " << f_.is_symmetric << std::endl;
+    std::cout << "pattern_symm " << f_.pattern_symm * 100 << "%" << std::endl;
+    std::cout << "value_symm   " << f_.value_symm * 100 << "%" << std::endl << std::endl;
+    std::cout << "num_rows   = " << f_.num_rows << std::endl;
+    std::cout << "num_cols   = " << f_.num_cols << std::endl;
+    std::cout << "nnz_mtx    = " << f_.nnz_mtx << std::endl;
+    std::cout << "real_nnz   = " << f_.num_nnzs << std::endl;
+    std::cout << "nnz_ratio  = " << f_.nnz_ratio * 100 << "%" << std::endl;
+    std::cout << "num_of_diags  = " << f_.complete_ndiags << std::endl;
+    std::cout << "nnz_closed2diag_ratio = " << f_.diag_close_ratio * 100 << "%" << std::endl << std::endl;
+    std::cout << "nnz_lower    : " << f_.nnz_lower << std::endl;
+    std::cout << "nnz_upper    : " << f_.nnz_upper << std::endl;
+    std::cout << "nnz_diagonal : " << f_.nnz_diagonal << std::endl << std::endl;
+    std::cout << "NoneZero_Row_Ratio      : " << f_.nz_row_ratio * 100 << "%" << std::endl;
+    std::cout << "min_nnz_each_row        : " << f_.min_nnz_each_row << std::endl;
+    std::cout << "max_nnz_each_row        : " << f_.max_nnz_each_row << std::endl;
+    std::cout << "ave_nnz_each_row        : " << f_.ave_nnz_each_row << std::endl;
+    std::cout << "var_nnz_each_row        : " << f_.var_nnz_each_row << std::endl;
+    std::cout << "standard_dev_row        : " << f_.standard_dev_row << std::endl;
+    std::cout << "P-ratio_row             : " << f_.P_ratio_row << std::endl;
+    std::cout << "Gini_coeff_row          : " << f_.Gini_row << std::endl << std::endl;
+    std::cout << "NoneZero_Col_Ratio      : " << f_.nz_col_ratio * 100 << "%" << std::endl;
+    std::cout << "min_nnz_each_col        : " << f_.min_nnz_each_col << std::endl;
+    std::cout << "max_nnz_each_col        : " << f_.max_nnz_each_col << std::endl;
+    std::cout << "ave_nnz_each_col        : " << f_.ave_nnz_each_col << std::endl;
+    std::cout << "var_nnz_each_col        : " << f_.var_nnz_each_col << std::endl;
+    std::cout << "standard_dev_col        : " << f_.standard_dev_col << std::endl;
+    std::cout << "P-ratio_col             : " << f_.P_ratio_col << std::endl;
+    std::cout << "Gini_coeff_col          : " << f_.Gini_col << std::endl << std::endl;
+    std::cout << "diagonal_dominant_ratio : " << f_.diagonal_dominant_ratio * 100 << "%" << std::endl << std::endl;
+    std::cout << "max_value_offdiag  = " << f_.max_value_offdiag << std::endl;
+    std::cout << "max_value_diagonal = " << f_.max_value_diagonal << std::endl;
+    std::cout << "row_variability    = " << f_.row_variability << std::endl;
+    std::cout << "col_variability    = " << f_.col_variability << std::endl << std::endl;
+    std::cout << "==========   Tile Features   ==========" << std::endl;
+    std::cout << "Number of Tiles  : " << f_.t_num_blocks << std::endl;
+    std::cout << "Tile inner size  : " << f_.t_num_RB << " * " << f_.t_num_CB << std::endl;
+    std::cout << "t_ave_nnz_tiles     : " << f_.t_ave_nnz_all_tiles << std::endl;
+    std::cout << "t_ave_nnz_RB        : " << f_.t_ave_nnz_RB << std::endl;
+    std::cout << "t_ave_nnz_CB        : " << f_.t_ave_nnz_CB << std::endl << std::endl;
+    std::cout << "t_var_nnz_tiles     : " << f_.t_var_nnz_all_tiles << std::endl;
+    std::cout << "t_var_nnz_RB        : " << f_.t_var_nnz_RB << std::endl;
+    std::cout << "t_var_nnz_CB        : " << f_.t_var_nnz_CB << std::endl << std::endl;
+    std::cout << "t_stand_dev_tiles     : " << f_.t_standard_dev_all_tiles << std::endl;
+    std::cout << "t_stand_dev_RB        : " << f_.t_standard_dev_RB << std::endl;
+    std::cout << "t_stand_dev_CB        : " << f_.t_standard_dev_CB << std::endl << std::endl;
+    std::cout << "min_nnz_each_tiles       : " << f_.t_min_nnz_all_tiles << std::endl;
+    std::cout << "max_nnz_each_tiles       : " << f_.t_max_nnz_all_tiles << std::endl;
+    std::cout << "min_nnz_each_RB          : " << f_.t_min_nnz_each_RB << std::endl;
+    std::cout << "max_nnz_each_RB          : " << f_.t_max_nnz_each_RB << std::endl;
+    std::cout << "min_nnz_each_CB          : " << f_.t_min_nnz_each_CB << std::endl;
+    std::cout << "max_nnz_each_CB          : " << f_.t_max_nnz_each_CB << std::endl << std::endl;
+    std::cout << "Gini_nnz_tiles     : " << f_.t_Gini_all_tiles << std::endl;
+    std::cout << "Gini_nnz_RB        : " << f_.t_Gini_RB << std::endl;
+    std::cout << "Gini_nnz_CB        : " << f_.t_Gini_CB << std::endl << std::endl;
+    std::cout << "P-ratio_tiles      : " << f_.t_P_ratio_all_tiles << std::endl;
+    std::cout << "P-ratio_RB         : " << f_.t_P_ratio_RB << std::endl;
+    std::cout << "P-ratio_CB         : " << f_.t_P_ratio_CB << std::endl << std::endl;
+    std::cout << "NE_ratio_tiles_     : " << f_.t_nz_ratio_tiles << std::endl;
+    std::cout << "NE_ratio_RB         : " << f_.t_nz_ratio_RB << std::endl;
+    std::cout << "NE_ratio_CB         : " << f_.t_nz_ratio_CB << std::endl << std::endl;
+  }
+
+  void ExtraFeaturesPrint() const {
+    std::cout << "==========   Tile Extra Features   ==========" << std::endl;
+    std::cout << "uniq_R         = " << f_.uniqR << std::endl;
+    std::cout << "uniq_C         = " << f_.uniqC << std::endl << std::endl;
+    std::cout << "GrX_uniqR      = " << f_.GrX_uniqR << std::endl;
+    std::cout << "GrX_uniqC      = " << f_.GrX_uniqC << std::endl << std::endl;
+    std::cout << "potReuseR      = " << f_.potReuseR << std::endl;
+    std::cout << "potReuseC      = " << f_.potReuseC << std::endl << std::endl;
+    std::cout << "GrX_potReuseR  = " << f_.GrX_potReuseR << std::endl;
+    std::cout << "GrX_potReuseC  = " << f_.GrX_potReuseC << std::endl << std::endl;
+    std::cout << "ave_distance_= " << f_.distance_per_row << std::endl << std::endl;
+  }
+
+  // <root>/<name>/<name>.features plus the .nnz/.max/.ave/.std and RB/CB images
+  // (src/sparse_features.cpp:1190-1405).
+  bool FeaturesWrite(const char* root) const {
+    const std::string dir = std::string(root) + "/" + matrixName;
+    const std::string mk = "mkdir -p '" + dir + "'";
+    if (std::system(mk.c_str()) != 0) return false;
+    const std::string base = dir + "/" + matrixName;
+    auto image = [&](const std::string& suffix, const auto& data) {
+      std::ofstream out(base + suffix);
+      for (const auto& v : data) out << v << std::endl;
+      return (bool)out;
+    };
+    bool ok = image(".ave", tile_ave_) && image(".max", tile_max_) && image(".nnz", tile_nnz_) &&
+              image(".std", tile_std_) && image(".RBmax", rb_max_) && image(".RBnnz", rb_nnz_) &&
+              image(".RBave", rb_ave_) && image(".RBstd", rb_std_) && image(".CBmax", cb_max_) &&
+              image(".CBnnz", cb_nnz_) && image(".CBave", cb_ave_) && image(".CBstd", cb_std_);
+    double v[LESPMV_NUM_FEATURES];
+    lespmv::check(lespmv_features_vector(&f_, v), "FeaturesWrite");
+    std::ofstream out(base + ".features");
+    out << "[" << matrixID_ << "]: " << matrixName << std::endl;
+    for (int i = 0; i < LESPMV_NUM_FEATURES; i++) {
+      const std::string name = lespmv_feature_name(i);
+      const bool integer = name == "num_rows" || name == "num_cols" || name == "num_nnz" ||
+                           name.rfind("max_nnz_each_", 0) == 0;
+      out << name << " ";
+      if (integer) out << (long long)v[i];
+      else out << v[i];
+      out << std::endl;
+    }
+    return ok && (bool)out;
+  }
+
+ private:
+  IndexType matrixID_ = 0;
+  std::string matrixName;
+  int32_t flags_ = 0;
+  std::shared_ptr<lespmv::Handle> handle_;
+  lespmv_features_t f_{};
+  std::vector<int32_t> tile_nnz_, tile_max_, rb_nnz_, rb_max_, cb_nnz_, cb_max_;
+  std::vector<double> tile_ave_, tile_std_, rb_ave_, rb_std_, cb_ave_, cb_std_;
+};

@@ -86,12 +86,16 @@ typedef struct {
   int32_t block_rows;  /* BSR br */
   int32_t block_cols;  /* BSR bc */
   int32_t omega;       /* CSR5 omega (lanes per tile) */
+  int32_t max_fill;    /* > 0: refuse (LESPMV_ERR_UNSUPPORTED) an ELL/sliced/DIA/BSR layout storing
+                          more than max_fill * nnz + num_rows slots; 0: no limit */
 } lespmv_params_t;
 
-/* Host-side CSR produced by lespmv_generate; arrays owned by the library. */
+/* Host-side CSR produced by lespmv_generate / lespmv_read_mtx /
+ * lespmv_coo_to_csr; arrays owned by the library (lespmv_host_csr_free). */
 typedef struct {
   int64_t m, n, nnz;
   int32_t dtype;
+  int32_t mm_flags; /* LESPMV_MM_SYMMETRIC when read from a symmetric MatrixMarket file */
   int32_t* row_offset;
   int32_t* col_index;
   void* values;
@@ -175,6 +179,8 @@ int lespmv_stream_create(lespmv_stream_t* s);
 int lespmv_stream_destroy(lespmv_stream_t s);
 int lespmv_stream_synchronize(lespmv_stream_t s);
 
+#define LESPMV_MM_SYMMETRIC 1
+
 /* MatrixMarket input, replacing read_csr_matrix (utils/sparse_io.cpp:179-204):
  * coordinate real/integer/pattern, symmetric expansion, then coo_to_csr with
  * optional duplicate compaction. Host arrays owned by the library. */
@@ -191,6 +197,74 @@ int lespmv_generate(int family, const double* params, int32_t nparams, uint64_t 
 int lespmv_host_csr_free(lespmv_host_csr_t* csr);
 /* x[i] = lo + (hi-lo) * u(splitmix64(seed ^ i)), u = (h >> 11) * 2^-53. */
 int lespmv_fill_uniform(void* host_dst, int dtype, int64_t n, uint64_t seed, double lo, double hi);
+
+/* ---- matrix features (Get_features) -------------------------------------------
+ * Replaces the MTX feature extractor: MtxLoad's histograms
+ * (src/sparse_features.cpp:121-604), P_ratioAndGini (:611-640),
+ * CalculateFeatures (:642-776), CalculateTilesFeatures (:782-853) and the
+ * distance of CalculateTilesExtraFeatures (:941-968), driven by
+ * test/Get_features.cpp:46-54, with IndexType int and ValueType double. The
+ * matrix is the handle's CSR read as a MatrixMarket entry list in row order;
+ * for a matrix read from a symmetric file pass LESPMV_FEAT_SYMMETRIC_FILE and the
+ * expanded CSR gives the reference's counts. Member names follow
+ * include/sparse_features.h:150-328. The 256 x 256 tile grid, the -1.0 start of
+ * the t_var_* and t_nz_ratio_* sums, and GrX_* = -1 (their code is compiled out
+ * in the reference) are kept. Counts are exact; ratios, p-ratio and Gini are
+ * computed from exact integer sums with the reference's formulas; variances
+ * from exact integer moments (the reference sums squared differences in file
+ * order, so these agree to rounding). */
+#define LESPMV_FEAT_SYMMETRIC_FILE 1
+#define LESPMV_FEAT_TILES 256 /* MAT_TILE_SIZE, include/general_config.h:8 */
+#define LESPMV_NUM_FEATURES 40 /* lines of the .features file, src/sparse_features.cpp:1347-1400 */
+
+typedef struct {
+  int64_t num_rows, num_cols, num_nnzs, nnz_mtx, nnz_lower, nnz_upper, nnz_diagonal, complete_ndiags, close_nnz;
+  int64_t min_nnz_each_row, max_nnz_each_row, min_nnz_each_col, max_nnz_each_col;
+  int64_t nz_rows, nz_cols, diag_dominant_rows;
+  int64_t t_num_blocks, t_num_RB, t_num_CB, t_mod_RB, t_mod_CB;
+  int64_t t_min_nnz_all_tiles, t_max_nnz_all_tiles, t_min_nnz_each_RB, t_max_nnz_each_RB;
+  int64_t t_min_nnz_each_CB, t_max_nnz_each_CB;
+  int64_t uniq_rows_sum, uniq_cols_sum, symm_pattern_pairs, symm_value_pairs;
+  int64_t is_symmetric;
+  double pattern_symm, value_symm, nnz_ratio, diag_close_ratio, distance_per_row;
+  double nz_row_ratio, ave_nnz_each_row, var_nnz_each_row, standard_dev_row, Gini_row, P_ratio_row;
+  double nz_col_ratio, ave_nnz_each_col, var_nnz_each_col, standard_dev_col, Gini_col, P_ratio_col;
+  double diagonal_dominant_ratio, max_value_offdiag, max_value_diagonal, row_variability, col_variability;
+  double t_ave_nnz_all_tiles, t_ave_nnz_RB, t_ave_nnz_CB, t_var_nnz_all_tiles, t_var_nnz_RB, t_var_nnz_CB;
+  double t_standard_dev_all_tiles, t_standard_dev_RB, t_standard_dev_CB;
+  double t_Gini_all_tiles, t_Gini_RB, t_Gini_CB, t_P_ratio_all_tiles, t_P_ratio_RB, t_P_ratio_CB;
+  double t_nz_ratio_tiles, t_nz_ratio_RB, t_nz_ratio_CB;
+  double uniqR, uniqC, potReuseR, potReuseC, GrX_uniqR, GrX_uniqC, GrX_potReuseR, GrX_potReuseC;
+  double device_ms; /* GPU time of the extraction */
+} lespmv_features_t;
+
+/* The images FeaturesWrite stores (src/sparse_features.cpp:1195-1338), plus the
+ * per-tile distinct-row / distinct-column counts (uniq_RB, uniq_CB). Host
+ * buffers; any pointer may be NULL. Tile arrays hold 256*256 entries in
+ * row-major tile order, block arrays 256. */
+typedef struct {
+  int32_t* tile_nnz;       /* .nnz  nnz_by_Tiles_ */
+  int32_t* tile_max;       /* .max  max_rownnz_per_tile_ */
+  double* tile_ave;        /* .ave  ave_rownnz_per_tile_ */
+  double* tile_std;        /* .std  std_rownnz_per_tile_ */
+  int32_t* tile_uniq_rows; /* uniq_RB */
+  int32_t* tile_uniq_cols; /* uniq_CB */
+  int32_t* rb_nnz;         /* .RBnnz */
+  int32_t* rb_max;         /* .RBmax */
+  double* rb_ave;          /* .RBave */
+  double* rb_std;          /* .RBstd */
+  int32_t* cb_nnz;         /* .CBnnz */
+  int32_t* cb_max;         /* .CBmax */
+  double* cb_ave;          /* .CBave */
+  double* cb_std;          /* .CBstd */
+} lespmv_feature_images_t;
+
+/* Computes every feature on the GPU (histogram and reduction kernels on the
+ * handle's stream); synchronous. `images` may be NULL. */
+int lespmv_features(lespmv_matrix_t A, int32_t flags, lespmv_features_t* out, lespmv_feature_images_t* images);
+/* The 40 values of a .features file in file order, and the name of value i. */
+int lespmv_features_vector(const lespmv_features_t* f, double out[LESPMV_NUM_FEATURES]);
+const char* lespmv_feature_name(int32_t i);
 
 #ifdef __cplusplus
 }

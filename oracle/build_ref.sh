@@ -20,10 +20,10 @@ SRCS=(
   "$REF/src/spmv_csr.cpp" "$REF/src/spmv_coo.cpp" "$REF/src/spmv_ell.cpp" "$REF/src/spmv_s_ell.cpp"
   "$REF/src/spmv_sell_c_sigma.cpp" "$REF/src/spmv_sell_c_R.cpp" "$REF/src/spmv_dia.cpp" "$REF/src/spmv_bsr.cpp"
   "$REF/src/spmv_csr5.cpp" "$REF/utils/thread.cpp" "$REF/utils/partition.cpp"
-  "$REF/utils/sparse_io.cpp" "$REF/utils/mmio.cpp"
+  "$REF/utils/sparse_io.cpp" "$REF/utils/mmio.cpp" "$REF/src/sparse_features.cpp"
 )
 "$CXX" -O2 -std=c++17 -fPIC -shared -fopenmp -march=native -w \
   -I"$REF/include" -I"$HERE" \
-  "$HERE/ref_driver.cpp" "${SRCS[@]}" \
+  "$HERE/ref_driver.cpp" "$HERE/ref_features.cpp" "${SRCS[@]}" \
   -Wl,-rpath,/opt/rocm/lib/llvm/lib -o "$OUT/libref_lespmv.so"
 echo "built $OUT/libref_lespmv.so"

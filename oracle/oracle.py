@@ -139,6 +139,22 @@ def reference_params(fmt: int, vdt) -> list:
     return [0, 0, 0, 0]
 
 
+IMAGE_DTYPES = {"tile_nnz": np.int32, "tile_max": np.int32, "tile_ave": np.float64, "tile_std": np.float64,
+                "tile_uniq_rows": np.int32, "tile_uniq_cols": np.int32, "rb_nnz": np.int32, "rb_max": np.int32,
+                "rb_ave": np.float64, "rb_std": np.float64, "cb_nnz": np.int32, "cb_max": np.int32,
+                "cb_ave": np.float64, "cb_std": np.float64, "row_nnz": np.int32, "col_nnz": np.int32}
+
+
+def feature_dicts(r: Result):
+    scal, img = {}, {}
+    for name in r.names():
+        if name.startswith("f:"):
+            scal[name[2:]] = float(r.get(name, np.float64)[0])
+        elif name.startswith("img:"):
+            img[name[4:]] = r.get(name, IMAGE_DTYPES[name[4:]])
+    return scal, img
+
+
 class Oracle:
     def __init__(self, path: str = ORACLE_SO):
         if not os.path.exists(path):
@@ -158,6 +174,7 @@ class Oracle:
         lib.oracle_time_csr_lb.restype = C.c_double
         lib.oracle_max_rel_error.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
         lib.oracle_max_rel_error.restype = C.c_double
+        lib.oracle_features.argtypes = [C.c_void_p, C.c_char_p]
         self.lib = lib
 
     def num_threads(self) -> int:
@@ -189,6 +206,13 @@ class Oracle:
         return self.lib.oracle_time_csr_lb(_dtype_id(v), m, n, len(ci), rp, ci, _ptr(v), _ptr(x), _ptr(y),
                                            threads, iters)
 
+    def features(self, path: str):
+        """The MTX features of a MatrixMarket file: (scalars, images)."""
+        r = Result(self.lib, "oracle")
+        if self.lib.oracle_features(r.h, path.encode()):
+            raise RuntimeError(f"oracle features failed: {r.error()}")
+        return feature_dicts(r)
+
     def max_rel_error(self, a, b) -> float:
         return self.lib.oracle_max_rel_error(_dtype_id(a), _ptr(a), _ptr(b), len(a))
 
@@ -208,6 +232,7 @@ class Reference:
                                        C.c_int]
         lib.ref_set_threads.argtypes = [C.c_int]
         lib.ref_read_mtx.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+        lib.ref_features.argtypes = [C.c_void_p, C.c_char_p]
         self.lib = lib
 
     def set_threads(self, t: int):
@@ -223,6 +248,13 @@ class Reference:
         r = Result(self.lib, "ref")
         self.lib.ref_read_mtx(r.h, 1 if np.dtype(dtype) == np.float64 else 0, path.encode(), int(compact))
         return r
+
+    def features(self, path: str):
+        """Get_features' values for a file (MTX<int, double>): (scalars, images)."""
+        r = Result(self.lib, "ref")
+        if self.lib.ref_features(r.h, path.encode()):
+            raise RuntimeError("reference features failed")
+        return feature_dicts(r)
 
     def coo_to_csr(self, m, n, ri, ci, v, compact=False) -> Result:
         r = Result(self.lib, "ref")

@@ -21,19 +21,7 @@ Matrix::~Matrix() {
 
 template <class F>
 int guarded(F&& f) {
-  try {
-    f();
-    return LESPMV_OK;
-  } catch (const Error& e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return LESPMV_ERR_OOM;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return LESPMV_ERR_DEVICE;
-  }
+  return guarded_call(std::forward<F>(f));
 }
 
 Matrix& mat(lespmv_matrix_t h) {

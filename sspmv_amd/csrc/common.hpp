@@ -35,6 +35,26 @@ struct Error : std::runtime_error {
 // Launch-error check after a kernel launch (asynchronous faults surface later).
 #define LSP_LAUNCHED() LSP_HIP(hipGetLastError())
 
+extern thread_local std::string g_last_error;
+
+// Runs f, turning exceptions into a status code and lespmv_last_error() text.
+template <class F>
+int guarded_call(F&& f) {
+  try {
+    f();
+    return LESPMV_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return LESPMV_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return LESPMV_ERR_DEVICE;
+  }
+}
+
 // Library-owned device buffer.
 struct DBuf {
   void* p = nullptr;
@@ -199,12 +219,22 @@ int64_t scan_exclusive_i64(int64_t* d, int64_t n, hipStream_t s);
 int64_t scan_exclusive_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, hipStream_t s);
 // Stable LSD radix sort of (key, value) uint32 pairs by the low `bits` key bits.
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, hipStream_t s);
+// LSD radix sort of uint32 keys by their low `bits` bits.
+void radix_sort_u32(uint32_t* keys, int64_t n, int bits, hipStream_t s);
 // Stable LSD radix sort of uint64 keys by their low `bits` bits.
 void radix_sort_u64(uint64_t* keys, int64_t n, int bits, hipStream_t s);
 int64_t reduce_max_rowlen(const int32_t* rp, int64_t m, hipStream_t s);
 void device_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 void coo_to_csr_device(int dtype, int64_t m, int64_t nnz, const int32_t* ri, const int32_t* ci, const void* v,
                        DBuf& rp_out, DBuf& ci_out, DBuf& v_out, hipStream_t s);
+
+// Refuses a padded layout of `slots` stored entries beyond p.max_fill * nnz + m.
+inline void check_fill(const lespmv_params_t& p, int64_t slots, int64_t nnz, int64_t m, const char* what) {
+  if (p.max_fill > 0 && (double)slots > (double)p.max_fill * (double)nnz + (double)m)
+    throw Error(LESPMV_ERR_UNSUPPORTED, std::string(what) + ": " + std::to_string(slots) +
+                                            " stored slots exceed the fill limit of " + std::to_string(p.max_fill) +
+                                            " x nnz");
+}
 
 // Host-side copies.
 template <class T>

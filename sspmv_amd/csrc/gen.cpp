@@ -340,6 +340,7 @@ extern "C" int lespmv_generate(int family, const double* prm, int32_t np, uint64
     out->n = B.n;
     out->nnz = (int64_t)B.ci.size();
     out->dtype = dtype;
+    out->mm_flags = 0;
     out->row_offset = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * (B.m + 1)));
     out->col_index = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * std::max<size_t>(B.ci.size(), 1)));
     out->values = std::malloc(lsp::vsize(dtype) * std::max<size_t>(B.val.size(), 1));

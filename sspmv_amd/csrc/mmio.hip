@@ -28,6 +28,7 @@ namespace {
 
 struct CooHost {
   int64_t m = 0, n = 0;
+  bool symmetric = false;
   std::vector<int32_t> ri, ci;
   std::vector<double> v;
 };
@@ -79,6 +80,7 @@ CooHost read_coo(const char* path) {
     c.v[i] = val;
   }
   if (symmetry == "symmetric") {
+    c.symmetric = true;
     std::vector<int32_t> ri, ci;
     std::vector<double> v;
     ri.reserve(2 * nnz);
@@ -162,6 +164,7 @@ void fill_out(lespmv_host_csr_t* out, int dtype, int64_t m, int64_t n, const std
   out->n = n;
   out->nnz = (int64_t)ci.size();
   out->dtype = dtype;
+  out->mm_flags = 0;
   out->row_offset = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * (m + 1)));
   out->col_index = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * std::max<size_t>(ci.size(), 1)));
   out->values = std::malloc(vsize(dtype) * std::max<size_t>(v.size(), 1));
@@ -254,6 +257,7 @@ extern "C" int lespmv_read_mtx(const char* path, int dtype, int compact, lespmv_
     std::vector<double> v;
     coo_to_csr_host(c, compact != 0, rp, ci, v);
     fill_out(out, dtype, c.m, c.n, rp, ci, v);
+    out->mm_flags = c.symmetric ? LESPMV_MM_SYMMETRIC : 0;
     return LESPMV_OK;
   } catch (const Error& e) {
     g_last_error = e.what();
@@ -305,6 +309,7 @@ extern "C" int lespmv_coo_to_csr(int dtype, int64_t m, int64_t n, int64_t nnz, c
     out->n = n;
     out->nnz = nnz;
     out->dtype = dtype;
+    out->mm_flags = 0;
     out->row_offset = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * (m + 1)));
     out->col_index = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
     out->values = std::malloc(vsize(dtype) * std::max<int64_t>(nnz, 1));

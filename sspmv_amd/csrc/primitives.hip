@@ -258,6 +258,10 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int bits, hipSt
   radix_sort_impl<uint32_t, true>(keys, vals, n, std::min(bits, 32), s);
 }
 
+void radix_sort_u32(uint32_t* keys, int64_t n, int bits, hipStream_t s) {
+  radix_sort_impl<uint32_t, false>(keys, nullptr, n, std::min(bits, 32), s);
+}
+
 void radix_sort_u64(uint64_t* keys, int64_t n, int bits, hipStream_t s) {
   radix_sort_impl<uint64_t, false>(keys, nullptr, n, std::min(bits, 64), s);
 }

@@ -244,6 +244,7 @@ void convert_bsr(Matrix& A, const lespmv_params_t& p, hipStream_t s) {
   }
   d->nnzb = scan_exclusive_i32_to_i64(flag.as<int32_t>(), pos.as<int64_t>(), nnz, s);
   LSP_REQUIRE(d->nnzb < INT32_MAX, LESPMV_ERR_UNSUPPORTED, "too many blocks for 32-bit block indices");
+  check_fill(p, d->nnzb * d->br * d->bc, A.nnz, A.m, "BSR layout");
   d->bcol.alloc(sizeof(int) * std::max<int64_t>(d->nnzb, 1));
   if (nnz) {
     hipLaunchKernelGGL(unique_scatter, grid1d(nnz, kBlock), dim3(kBlock), 0, s, nnz, d->nb, keys.as<uint64_t>(),

@@ -246,6 +246,7 @@ void convert_dia(Matrix& A, const lespmv_params_t& p, hipStream_t s) {
   LSP_LAUNCHED();
   const int64_t nd = scan_exclusive_i32_to_i64(flag.as<int32_t>(), pos.as<int64_t>(), len, s);
   d->ndiag = (int32_t)nd;
+  check_fill(p, nd * align_up(A.m, p.alignment), A.nnz, A.m, "DIA layout");
   if (nd > p.max_diags) {
     throw Error(LESPMV_ERR_MAX_DIAGS,
                 "Number of diagonals (" + std::to_string(nd) + ") excedes limit (" + std::to_string(p.max_diags) + ")");

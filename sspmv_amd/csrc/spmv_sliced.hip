@@ -318,6 +318,7 @@ void convert_sliced(Matrix& A, int fmt, const lespmv_params_t& p, hipStream_t s)
                          s));
   LSP_HIP(hipStreamSynchronize(s));
   d->max_width = ell ? (int32_t)(d->nchunks ? reduce_max_rowlen(A.rp.as<int32_t>(), m, s) : 0) : 0;
+  check_fill(p, d->slots, A.nnz, m, "sliced layout");
   d->col.alloc(sizeof(int) * d->slots);
   d->val.alloc(vsize(A.dtype) * d->slots);
   LSP_HIP(hipMemsetAsync(d->col.p, 0xFF, sizeof(int) * d->slots, s));  // pads: col -1

@@ -35,7 +35,7 @@ class LespmvError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("sigma", C.c_int32), ("alignment", C.c_int32), ("ld", C.c_int32),
                 ("max_diags", C.c_int32), ("block_rows", C.c_int32), ("block_cols", C.c_int32),
-                ("omega", C.c_int32)]
+                ("omega", C.c_int32), ("max_fill", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -43,8 +43,47 @@ class Params(C.Structure):
 
 class HostCsrC(C.Structure):
     _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64), ("dtype", C.c_int32),
-                ("row_offset", C.POINTER(C.c_int32)), ("col_index", C.POINTER(C.c_int32)),
+                ("mm_flags", C.c_int32), ("row_offset", C.POINTER(C.c_int32)), ("col_index", C.POINTER(C.c_int32)),
                 ("values", C.c_void_p)]
+
+
+# lespmv_features_t, field for field (include/lespmv_hip.h)
+FEATURE_INT_FIELDS = [
+    "num_rows", "num_cols", "num_nnzs", "nnz_mtx", "nnz_lower", "nnz_upper", "nnz_diagonal", "complete_ndiags",
+    "close_nnz", "min_nnz_each_row", "max_nnz_each_row", "min_nnz_each_col", "max_nnz_each_col", "nz_rows",
+    "nz_cols", "diag_dominant_rows", "t_num_blocks", "t_num_RB", "t_num_CB", "t_mod_RB", "t_mod_CB",
+    "t_min_nnz_all_tiles", "t_max_nnz_all_tiles", "t_min_nnz_each_RB", "t_max_nnz_each_RB", "t_min_nnz_each_CB",
+    "t_max_nnz_each_CB", "uniq_rows_sum", "uniq_cols_sum", "symm_pattern_pairs", "symm_value_pairs",
+    "is_symmetric"]
+FEATURE_DOUBLE_FIELDS = [
+    "pattern_symm", "value_symm", "nnz_ratio", "diag_close_ratio", "distance_per_row", "nz_row_ratio",
+    "ave_nnz_each_row", "var_nnz_each_row", "standard_dev_row", "Gini_row", "P_ratio_row", "nz_col_ratio",
+    "ave_nnz_each_col", "var_nnz_each_col", "standard_dev_col", "Gini_col", "P_ratio_col",
+    "diagonal_dominant_ratio", "max_value_offdiag", "max_value_diagonal", "row_variability", "col_variability",
+    "t_ave_nnz_all_tiles", "t_ave_nnz_RB", "t_ave_nnz_CB", "t_var_nnz_all_tiles", "t_var_nnz_RB", "t_var_nnz_CB",
+    "t_standard_dev_all_tiles", "t_standard_dev_RB", "t_standard_dev_CB", "t_Gini_all_tiles", "t_Gini_RB",
+    "t_Gini_CB", "t_P_ratio_all_tiles", "t_P_ratio_RB", "t_P_ratio_CB", "t_nz_ratio_tiles", "t_nz_ratio_RB",
+    "t_nz_ratio_CB", "uniqR", "uniqC", "potReuseR", "potReuseC", "GrX_uniqR", "GrX_uniqC", "GrX_potReuseR",
+    "GrX_potReuseC", "device_ms"]
+FEAT_SYMMETRIC_FILE = 1
+FEAT_TILES = 256
+NUM_FEATURES = 40
+
+
+class FeaturesC(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in FEATURE_INT_FIELDS] + [(n, C.c_double) for n in FEATURE_DOUBLE_FIELDS]
+
+
+# lespmv_feature_images_t: (name, element type, length)
+FEATURE_IMAGES = [("tile_nnz", np.int32, 65536), ("tile_max", np.int32, 65536), ("tile_ave", np.float64, 65536),
+                  ("tile_std", np.float64, 65536), ("tile_uniq_rows", np.int32, 65536),
+                  ("tile_uniq_cols", np.int32, 65536), ("rb_nnz", np.int32, 256), ("rb_max", np.int32, 256),
+                  ("rb_ave", np.float64, 256), ("rb_std", np.float64, 256), ("cb_nnz", np.int32, 256),
+                  ("cb_max", np.int32, 256), ("cb_ave", np.float64, 256), ("cb_std", np.float64, 256)]
+
+
+class FeatureImagesC(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n, _, _ in FEATURE_IMAGES]
 
 
 _lib = None
@@ -96,6 +135,9 @@ def load() -> C.CDLL:
         "lespmv_fill_uniform": ([vp, C.c_int, i64, C.c_uint64, dbl, dbl], C.c_int),
         "lespmv_read_mtx": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(HostCsrC)], C.c_int),
         "lespmv_coo_to_csr": ([C.c_int, i64, i64, i64, vp, vp, vp, C.c_int, C.POINTER(HostCsrC)], C.c_int),
+        "lespmv_features": ([vp, i32, C.POINTER(FeaturesC), C.POINTER(FeatureImagesC)], C.c_int),
+        "lespmv_features_vector": ([C.POINTER(FeaturesC), C.POINTER(dbl)], C.c_int),
+        "lespmv_feature_name": ([i32], C.c_char_p),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -191,6 +233,7 @@ class HostCsr:
     rp: np.ndarray
     ci: np.ndarray
     v: np.ndarray
+    symmetric: bool = False  # read from a symmetric MatrixMarket file
 
     @property
     def nnz(self) -> int:
@@ -207,7 +250,7 @@ def _take_host_csr(out: HostCsrC) -> HostCsr:
         v = np.ctypeslib.as_array(C.cast(out.values, C.POINTER(vt)), shape=(max(nnz, 1),))[:nnz].copy()
     finally:
         lib.lespmv_host_csr_free(C.byref(out))
-    return HostCsr(int(out.m), int(out.n), rp, ci, v)
+    return HostCsr(int(out.m), int(out.n), rp, ci, v, bool(out.mm_flags & 1))
 
 
 def generate(family: int, params, seed: int, dtype=np.float64) -> HostCsr:
@@ -328,8 +371,35 @@ class Matrix:
         check(lib.lespmv_export(self.h, fmt, name.encode(), out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
 
+    def features(self, symmetric: bool = False, images: bool = False):
+        return features(self, symmetric, images)
+
     def export_scalar(self, fmt: int, name: str) -> int:
         return int(self.export(fmt, name, np.int64)[0])
+
+
+def feature_names() -> list:
+    """The 40 line names of a .features file, in file order."""
+    lib = load()
+    return [lib.lespmv_feature_name(i).decode() for i in range(NUM_FEATURES)]
+
+
+def features(M: "Matrix", symmetric: bool = False, images: bool = False):
+    """GPU features of a handle: (dict of lespmv_features_t fields, 40-vector,
+    dict of images or None)."""
+    lib = load()
+    f = FeaturesC()
+    img = None
+    imgc = None
+    if images:
+        img = {n: np.zeros(k, dtype=t) for n, t, k in FEATURE_IMAGES}
+        imgc = FeatureImagesC(*[img[n].ctypes.data_as(C.c_void_p) for n, _, _ in FEATURE_IMAGES])
+    check(lib.lespmv_features(M.h, FEAT_SYMMETRIC_FILE if symmetric else 0, C.byref(f),
+                              C.byref(imgc) if imgc is not None else None))
+    vec = np.zeros(NUM_FEATURES)
+    check(lib.lespmv_features_vector(C.byref(f), vec.ctypes.data_as(C.POINTER(C.c_double))))
+    vals = {n: getattr(f, n) for n, _ in FeaturesC._fields_}
+    return vals, vec, img
 
 
 def time_copy(dst: DeviceArray, src: DeviceArray, nbytes: int, iters: int) -> float:

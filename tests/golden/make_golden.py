@@ -27,9 +27,10 @@ sys.path.insert(0, ROOT)
 
 from oracle.oracle import (BSR, COO, CSR, CSR5, DIA, ELL, SCALARS, SELL, SELL_C_R, SELL_C_SIGMA,  # noqa: E402
                            Reference, array_dtypes, reference_params)
-from tests.matrices import suite  # noqa: E402
+from tests.matrices import feature_cases, suite  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lespmv_golden.npz")
+FEATURES_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "features_golden.npz")
 
 
 # FP32 fixtures are kept for a subset to keep the file small; FP64 covers all.
@@ -180,5 +181,30 @@ def main():
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT) / 1e6:.2f} MB")
 
 
+def write_features():
+    """Get_features (MTX<int, double>) on the MatrixMarket texts of
+    tests.matrices.feature_cases(), which the tests regenerate byte for byte.
+    Stores every feature scalar and every image array."""
+    import tempfile
+    ref = Reference()
+    ref.set_threads(4)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name, text in feature_cases().items():
+            path = os.path.join(d, name + ".mtx")
+            with open(path, "w") as f:
+                f.write(text)
+            scal, img = ref.features(path)
+            names = sorted(scal)
+            out[key(name, "scalar_names")] = np.array(names)
+            out[key(name, "scalar_values")] = np.array([scal[k] for k in names], dtype=np.float64)
+            for k, arr in img.items():
+                out[key(name, "img", k)] = arr
+    np.savez_compressed(FEATURES_OUT, **out)
+    print(f"wrote {FEATURES_OUT}: {len(out)} arrays, {os.path.getsize(FEATURES_OUT) / 1e6:.2f} MB")
+
+
 if __name__ == "__main__":
-    main()
+    if "--features-only" not in sys.argv:
+        main()
+    write_features()
