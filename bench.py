@@ -11,7 +11,9 @@ One step = one y = A*x with the matrix and vectors resident in HBM.
 
 Prints ONE JSON line. `value` is the effective bandwidth (algorithmic bytes:
 values + indices + x read once + y written) of the fastest kernel (least time)
-on the FP64 lognormal matrix, divided by the wall time per step. `roofline` compares the
+on the FP64 lognormal matrix, divided by the wall time per step. `serving`
+holds the batched workload (sspmv_amd/bin/serve_bench): requests/s and p50 /
+p99 latency of the packed persistent kernel against plain per-call CSR. `roofline` compares the
 same bytes with that kernel's device time (hipEvents on its stream) against
 MI355X's 8 TB/s. `cpu_baseline` times the CPU oracle's kernel_flag=2 CSR SpMV
 (the reference's load-balanced OpenMP path, restated) on this host's cores.
@@ -106,6 +108,25 @@ def cpu_baseline(A, x, threads, max_seconds):
     return ms, iters, y
 
 
+def serving(matrices, requests, rates):
+    """The batched serving workload (sspmv_amd/bin/serve_bench): 256 resident
+    matrices, requests with k in 1..32, packed persistent kernel vs per-call CSR."""
+    import subprocess
+    exe = os.path.join(ROOT, "sspmv_amd", "bin", "serve_bench")
+    cmd = [exe, f"--matrices={matrices}", f"--requests={requests}", f"--rates={rates}"]
+    log("[serving] " + " ".join(cmd[1:]))
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    for line in out.stderr.splitlines():
+        log("[serving] " + line)
+    if out.returncode != 0:
+        raise RuntimeError(f"serve_bench failed ({out.returncode}): {out.stderr[-2000:]}")
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    sat = r["server_saturation"]["req_per_s"]
+    r["speedup_vs_percall_async"] = round(sat / r["percall_async"]["req_per_s"], 3)
+    r["speedup_vs_percall_sync"] = round(sat / r["percall_sync"]["req_per_s"], 3)
+    return r
+
+
 def copy_bandwidth(lib, nbytes=1 << 30, iters=20):
     src = lib.DeviceArray(nbytes)
     dst = lib.DeviceArray(nbytes)
@@ -123,7 +144,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bound on the CPU baseline's timed work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--formats", default="all", help="comma list of kernel labels to measure, or 'all'")
-    ap.add_argument("--sections", default="lognormal,lognormal_fp32,stencil27,copy")
+    ap.add_argument("--sections", default="lognormal,lognormal_fp32,stencil27,copy,serving")
+    ap.add_argument("--serve-matrices", type=int, default=256)
+    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-rates", default="5000,20000,50000")
     ap.add_argument("--copy-only", action="store_true", help="only run the plain copy kernel (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -165,7 +189,7 @@ def main():
     y_gpu = dy.numpy()
 
     out = {
-        "metric": "SpMV effective GB/s (values+indices+x read once+y) and fraction of HBM peak",
+        "metric": "SpMV GB/s & %HBM peak; batched req/s + p99 latency vs plain CSR, same GPU",
         "value": round(eff / (wall_ms * 1e-3) / 1e9, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -224,6 +248,9 @@ def main():
         dx = lib.DeviceArray.from_numpy(lib.fill_uniform(S.n, 5))
         dy = lib.DeviceArray.empty(S.m, np.float64)
         out["formats"]["stencil27_f64"] = summary(measure(lib, M, dx, dy, "stencil27", wanted, args.warmup))
+        del M, dx, dy, S
+    if "serving" in sections:
+        out["serving"] = serving(args.serve_matrices, args.serve_requests, args.serve_rates)
     print(json.dumps(out), flush=True)
 
 

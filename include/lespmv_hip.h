@@ -198,6 +198,95 @@ int lespmv_host_csr_free(lespmv_host_csr_t* csr);
 /* x[i] = lo + (hi-lo) * u(splitmix64(seed ^ i)), u = (h >> 11) * 2^-53. */
 int lespmv_fill_uniform(void* host_dst, int dtype, int64_t n, uint64_t seed, double lo, double hi);
 
+/* ---- exhaustive timing and autotuning ------------------------------------------
+ * No reference counterpart (the reference times one format per driver run,
+ * test/benchmark_spmv_<fmt>.cpp, and collects the results in spreadsheets).
+ * A candidate is a (format, variant, parameters) triple; lespmv_tune converts
+ * each one on the device, times `iters` SpMV (k = 1) or SpMM (k <= 32) calls
+ * after two warm-up calls with hipEvents, and releases it again. Candidates
+ * that do not apply (too many diagonals, padding beyond params.max_fill * nnz,
+ * CSR5 limits) get a non-zero status and ms = +inf. */
+typedef struct {
+  int32_t fmt;
+  int32_t variant;
+  lespmv_params_t params;
+  int32_t status;     /* LESPMV_OK or the conversion / timing error */
+  double ms;          /* device ms per call */
+  double convert_ms;  /* device ms of the conversion */
+} lespmv_tune_result_t;
+
+/* The standard grid for the formats whose bit is set in fmt_mask (bit = format
+ * id): CSR lanes per row 2..64 and merge-path, COO, ELL, S-ELL / SELL-C-sigma
+ * (sigma 256, 4096) / SELL-C-R with C in {32, 64}, DIA both variants, BSR
+ * 2x2 4x4 8x4 8x8 16x4, CSR5 (omega, sigma) in {(64,16), (32,16), (64,8)}.
+ * With out == NULL only *count is set. */
+int lespmv_tune_candidates(int32_t fmt_mask, int32_t dtype, lespmv_tune_result_t* out, int32_t cap, int32_t* count);
+int lespmv_tune(lespmv_matrix_t A, lespmv_tune_result_t* candidates, int32_t n, int32_t iters, int32_t k);
+/* Best candidate of one format's grid; the winner stays converted on A. */
+int lespmv_autotune(lespmv_matrix_t A, int32_t fmt, int32_t iters, int32_t k, lespmv_tune_result_t* best);
+
+/* ---- MLP trained on the GPU (the learned selector's network) -------------------
+ * No reference counterpart (the reference's selector is a Keras model,
+ * SmartAdpter/models/SpMV_Adapter.py:29-154). in -> hidden1 -> hidden2 ->
+ * sum(head_sizes), ReLU, one softmax cross-entropy per head (label -1 = none),
+ * Adam. FP32. Inputs are standardised with the stored mean / std. Host
+ * buffers throughout; X is n x in row-major, labels n x nheads. */
+typedef struct lespmv_mlp_s* lespmv_mlp_t;
+int lespmv_mlp_create(lespmv_mlp_t* out, int32_t in_dim, int32_t hidden1, int32_t hidden2, int32_t nheads,
+                      const int32_t* head_sizes, uint64_t seed);
+int lespmv_mlp_destroy(lespmv_mlp_t mlp);
+int lespmv_mlp_shape(lespmv_mlp_t mlp, int32_t* in_dim, int32_t* out_dim, int64_t* nparams);
+/* Flat parameters: W1 (hidden1 x in), b1, W2 (hidden2 x hidden1), b2, W3 (out x hidden2), b3. */
+int lespmv_mlp_get_params(lespmv_mlp_t mlp, float* host);
+int lespmv_mlp_set_params(lespmv_mlp_t mlp, const float* host);
+int lespmv_mlp_set_normalizer(lespmv_mlp_t mlp, const float* mean, const float* stddev);
+/* Loss (sum over heads of the mean cross-entropy of the labelled samples) and
+ * its gradient with respect to the flat parameters, over the whole batch. */
+int lespmv_mlp_loss_grad(lespmv_mlp_t mlp, const float* X, const int32_t* labels, int32_t n, double* loss,
+                         float* grad);
+/* Mini-batch Adam (beta 0.9 / 0.999, eps 1e-8, decoupled weight decay); epoch_loss[epochs] may be NULL. */
+int lespmv_mlp_train(lespmv_mlp_t mlp, const float* X, const int32_t* labels, int32_t n, int32_t epochs,
+                     int32_t batch, float lr, float weight_decay, uint64_t seed, float* epoch_loss);
+/* Per-head softmax probabilities, n x out. */
+int lespmv_mlp_forward(lespmv_mlp_t mlp, const float* X, int32_t n, float* probs);
+int lespmv_mlp_save(lespmv_mlp_t mlp, const char* path);
+int lespmv_mlp_load(lespmv_mlp_t* out, const char* path);
+
+
+/* ---- batched serving --------------------------------------------------------------
+ * No reference counterpart. A server keeps its own CSR copies of resident
+ * matrices within an HBM budget (admission by footprint: adding a matrix evicts
+ * idle matrices, least recently used first; LESPMV_ERR_OOM when the matrices in
+ * use leave no room). Requests (y = alpha A x + beta y for k = 1..32 right-hand
+ * sides, row-major device X / Y) are queued; every call to submit / poll
+ * retires finished batches and packs the queued requests, up to max_batch per
+ * batch, into one launch of a persistent kernel replayed from a HIP graph on a
+ * free slot (up to nslots batches in flight on their own streams).
+ * A request's result does not depend on the batch it ran in. Latency is
+ * measured on the host from submit to the poll that sees the batch finished.
+ * Requests that write the same Y must not be in flight together. */
+typedef struct lespmv_server_s* lespmv_server_t;
+typedef struct {
+  int32_t resident, queued;
+  int64_t bytes_used, bytes_budget, batches, items, completed;
+} lespmv_server_stats_t;
+int lespmv_server_create(lespmv_server_t* out, int32_t dtype, int64_t hbm_budget, int32_t max_batch, int32_t nslots);
+int lespmv_server_destroy(lespmv_server_t S);
+/* Copies a host CSR in; lanes = rows-per-wave setting (power of two <= 64), 0 = autotune. */
+int lespmv_server_add_csr(lespmv_server_t S, int64_t m, int64_t n, int64_t nnz, const int32_t* row_offset,
+                          const int32_t* col_index, const void* values, int32_t lanes, int32_t* id);
+int lespmv_server_remove(lespmv_server_t S, int32_t id);
+/* The resident matrix as a handle (owned by the server) and its lanes setting. */
+int lespmv_server_matrix(lespmv_server_t S, int32_t id, lespmv_matrix_t* A, int32_t* lanes);
+int lespmv_server_submit(lespmv_server_t S, int32_t id, int32_t k, double alpha, const void* X, int64_t ldx,
+                         double beta, void* Y, int64_t ldy, int64_t* request_id);
+/* Completed requests since the last poll (wait = 1 blocks until at least one
+ * completes or nothing is in flight). */
+int lespmv_server_poll(lespmv_server_t S, int32_t wait, int64_t* request_ids, double* latency_us, int32_t cap,
+                       int32_t* ndone);
+int lespmv_server_drain(lespmv_server_t S);
+int lespmv_server_stats(lespmv_server_t S, lespmv_server_stats_t* stats);
+
 /* ---- matrix features (Get_features) -------------------------------------------
  * Replaces the MTX feature extractor: MtxLoad's histograms
  * (src/sparse_features.cpp:121-604), P_ratioAndGini (:611-640),
@@ -265,6 +354,31 @@ int lespmv_features(lespmv_matrix_t A, int32_t flags, lespmv_features_t* out, le
 /* The 40 values of a .features file in file order, and the name of value i. */
 int lespmv_features_vector(const lespmv_features_t* f, double out[LESPMV_NUM_FEATURES]);
 const char* lespmv_feature_name(int32_t i);
+/* Number of non-empty br x bc blocks for each (br, bc) pair in dims[2*n]
+ * (the BSR block count without building BSR); selector input, no reference
+ * counterpart. Synchronous. */
+int lespmv_block_counts(lespmv_matrix_t A, const int32_t* dims, int32_t n, int64_t* nnzb);
+
+/* ---- learned format selector ----------------------------------------------------
+ * Replaces the reference's MM-Adapter prediction (SmartAdpter/models/
+ * SpMV_Adapter.py:90-122) with an MLP trained on this library's own timings.
+ * 128 inputs from the GPU features (the 40 .features values, 24 derived
+ * scalars incl. BSR block fills, 32-bin row-block and column-block shares).
+ * Six heads (32 outputs): format class {CSR-vector, CSR-merge, COO, ELL,
+ * SELL family, DIA, BSR, CSR5}, CSR lanes, SELL (C, sigma), DIA variant, BSR
+ * block, CSR5 (omega, sigma); a class's parameter head indexes its run
+ * [class_first[c], class_first[c+1]) of the standard tuning grid (all formats).
+ * allowed[32] marks what the features already rule out (ELL / DIA fill,
+ * diagonal count, BSR fill of the counted block sizes). */
+#define LESPMV_SELECTOR_INPUTS 128
+#define LESPMV_SELECTOR_OUTPUTS 32
+int lespmv_selector_layout(int32_t* class_first /*[9]*/, int32_t* head_of_class /*[8]*/, int32_t* head_sizes /*[6]*/);
+int lespmv_selector_inputs(lespmv_matrix_t A, int32_t flags, float* inputs /*[128]*/, uint8_t* allowed /*[32]*/,
+                           lespmv_features_t* features /* may be NULL */);
+int lespmv_selector_decode(const float* probs /*[32]*/, const uint8_t* allowed /*[32]*/, int32_t* cls,
+                           int32_t* candidate);
+/* Features, forward pass and decode: the chosen grid candidate (not converted). */
+int lespmv_select(lespmv_matrix_t A, lespmv_mlp_t mlp, int32_t flags, lespmv_tune_result_t* choice, int32_t* cls);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,270 @@
+// serve_bench: batched serving against plain per-call CSR on one GPU.
+//
+// Builds the batched workload (matrices with rows log-uniform in [1k, 200k]
+// drawn from 2-D / 3-D stencils, lognormal-row random matrices and 4x4-block
+// random matrices with 8 blocks per block row; FP64), makes them resident in a
+// lespmv server (lanes per row autotuned per matrix), and draws a request
+// stream: matrix uniform, k uniform in {1..32}. It then measures
+//   * the server at saturation (every request queued at once) and under
+//     Poisson arrivals at the given rates: requests/s and p50 / p99 latency
+//     (submit to completion, host clock);
+//   * plain per-call CSR on the same requests: lespmv_spmm with the default
+//     CSR kernel, (a) one synchronous call per request (the reference's
+//     LeSpMV_csr usage), (b) all calls issued asynchronously on the matrices'
+//     own streams and one synchronisation at the end (its throughput bound).
+// Prints one JSON object on stdout, progress on stderr.
+// Options: --matrices=N --requests=N --rates=r1,r2,... --seed=S --budget-gb=G
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../../include/lespmv_hip.h"
+
+static std::string arg(int argc, char** argv, const char* key, const char* def) {
+  const size_t n = std::strlen(key);
+  for (int i = 1; i < argc; i++)
+    if (std::strncmp(argv[i], key, n) == 0 && argv[i][n] == '=') return argv[i] + n + 1;
+  return def;
+}
+
+#define CHECK(call)                                                                   \
+  do {                                                                                \
+    int rc_ = (call);                                                                 \
+    if (rc_ != LESPMV_OK) {                                                           \
+      std::fprintf(stderr, "error %d at %s: %s\n", rc_, #call, lespmv_last_error()); \
+      std::exit(3);                                                                   \
+    }                                                                                 \
+  } while (0)
+
+static double now_us() {
+  return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static uint64_t mix(uint64_t x) {
+  uint64_t z = x + 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+static double unit(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+static double pct(std::vector<double> v, double q) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  return v[std::min(v.size() - 1, (size_t)std::floor(q * (double)(v.size() - 1) + 0.5))];
+}
+static double mean(const std::vector<double>& v) {
+  return v.empty() ? 0.0 : std::accumulate(v.begin(), v.end(), 0.0) / (double)v.size();
+}
+
+struct Mat {
+  int32_t id = -1;
+  lespmv_matrix_t h = nullptr;
+  int64_t m = 0, n = 0, nnz = 0;
+  void *X = nullptr, *Y = nullptr, *Y2 = nullptr;
+  int fam = 0;
+};
+
+struct Req {
+  int mat, k;
+  double arrival_us;
+};
+
+int main(int argc, char** argv) {
+  const int nmat = std::stoi(arg(argc, argv, "--matrices", "256"));
+  const int nreq = std::stoi(arg(argc, argv, "--requests", "4000"));
+  const uint64_t seed = std::stoull(arg(argc, argv, "--seed", "1000"));
+  const double budget_gb = std::stod(arg(argc, argv, "--budget-gb", "200"));
+  std::vector<double> rates;
+  {
+    const std::string r = arg(argc, argv, "--rates", "5000,20000,50000");
+    size_t p = 0;
+    while (p < r.size()) {
+      size_t q = r.find(',', p);
+      if (q == std::string::npos) q = r.size();
+      rates.push_back(std::stod(r.substr(p, q - p)));
+      p = q + 1;
+    }
+  }
+  CHECK(lespmv_init(0));
+  lespmv_server_t S;
+  CHECK(lespmv_server_create(&S, LESPMV_F64, (int64_t)(budget_gb * 1e9), 128, 3));
+
+  // ---- resident matrices and per-call handles ----
+  std::vector<Mat> mats(nmat);
+  int64_t total_nnz = 0, total_rows = 0;
+  const double t_build = now_us();
+  for (int i = 0; i < nmat; i++) {
+    const uint64_t h = mix(seed * 7919 + (uint64_t)i);
+    const double rows = std::exp(std::log(1e3) + unit(h) * (std::log(2e5) - std::log(1e3)));
+    const int fam = (int)(mix(h) % 4);
+    std::vector<double> p;
+    int gen;
+    if (fam == 0) {
+      const double nx = std::max(8.0, std::floor(std::sqrt(rows)));
+      gen = LESPMV_GEN_LAPLACE2D;
+      p = {nx, nx, 4.0, -1.0};
+    } else if (fam == 1) {
+      const double k = std::max(4.0, std::floor(std::cbrt(rows)));
+      gen = LESPMV_GEN_STENCIL27;
+      p = {k, 26.0, -1.0};
+    } else if (fam == 2) {
+      gen = LESPMV_GEN_LOGNORMAL;
+      p = {std::floor(rows), std::floor(rows), 16.0, 1.0, 4096, 0.3, 64};
+    } else {
+      const double mb = std::max(1.0, std::floor(rows / 4));
+      gen = LESPMV_GEN_BLOCKRAND;
+      p = {mb, mb, 4, 4, 8};
+    }
+    lespmv_host_csr_t c;
+    CHECK(lespmv_generate(gen, p.data(), (int32_t)p.size(), 1000 + (uint64_t)i, LESPMV_F64, &c));
+    Mat& M = mats[i];
+    M.m = c.m;
+    M.n = c.n;
+    M.nnz = c.nnz;
+    M.fam = fam;
+    CHECK(lespmv_server_add_csr(S, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, 0, &M.id));
+    CHECK(lespmv_create_csr(&M.h, LESPMV_F64, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, 0));
+    lespmv_host_csr_free(&c);
+    CHECK(lespmv_malloc(&M.X, 8 * M.n * 32));
+    CHECK(lespmv_malloc(&M.Y, 8 * M.m * 32));
+    CHECK(lespmv_malloc(&M.Y2, 8 * M.m * 32));
+    std::vector<double> x((size_t)M.n * 32);
+    CHECK(lespmv_fill_uniform(x.data(), LESPMV_F64, (int64_t)x.size(), 77 + (uint64_t)i, -1.0, 1.0));
+    CHECK(lespmv_memcpy_h2d(M.X, x.data(), 8 * (int64_t)x.size()));
+    total_nnz += M.nnz;
+    total_rows += M.m;
+    if ((i + 1) % 32 == 0) std::fprintf(stderr, "built %d/%d matrices\n", i + 1, nmat);
+  }
+  CHECK(lespmv_synchronize());
+  lespmv_server_stats_t st;
+  CHECK(lespmv_server_stats(S, &st));
+  const double build_s = (now_us() - t_build) * 1e-6;
+
+  // ---- request stream ----
+  std::vector<Req> reqs(nreq);
+  for (int r = 0; r < nreq; r++) {
+    const uint64_t h = mix(seed * 104729 + (uint64_t)r);
+    reqs[r].mat = (int)(h % (uint64_t)nmat);
+    reqs[r].k = 1 + (int)(mix(h) % 32);
+  }
+  int64_t flops = 0;
+  for (const auto& q : reqs) flops += 2 * mats[q.mat].nnz * q.k;
+
+  auto run_server = [&](double rate) {
+    // Poisson arrivals at `rate` req/s, or everything at t = 0 (saturation)
+    std::mt19937_64 rng(seed + (uint64_t)rate);
+    std::exponential_distribution<double> gap(rate > 0 ? rate * 1e-6 : 1.0);
+    double t = 0;
+    for (auto& q : reqs) {
+      q.arrival_us = rate > 0 ? t : 0.0;
+      if (rate > 0) t += gap(rng);
+    }
+    std::vector<double> lat;
+    lat.reserve(nreq);
+    std::vector<int64_t> ids(4096);
+    std::vector<double> l(4096);
+    const double t0 = now_us();
+    int next = 0, done = 0;
+    while (done < nreq) {
+      const double now = now_us() - t0;
+      while (next < nreq && reqs[next].arrival_us <= now) {
+        const Req& q = reqs[next];
+        Mat& M = mats[q.mat];
+        void* Y = (next & 1) ? M.Y2 : M.Y;  // results are checked by the tests, not here
+        CHECK(lespmv_server_submit(S, M.id, q.k, 1.0, M.X, q.k, 0.0, Y, q.k, nullptr));
+        next++;
+      }
+      int32_t n = 0;
+      CHECK(lespmv_server_poll(S, next == nreq ? 1 : 0, ids.data(), l.data(), (int32_t)ids.size(), &n));
+      for (int i = 0; i < n; i++) lat.push_back(l[i]);
+      done += n;
+    }
+    const double secs = (now_us() - t0) * 1e-6;
+    char buf[512];
+    std::snprintf(buf, sizeof buf,
+                  "{\"offered_req_per_s\": %s, \"req_per_s\": %.1f, \"p50_us\": %.1f, \"p99_us\": %.1f, "
+                  "\"mean_us\": %.1f, \"gflops\": %.1f}",
+                  rate > 0 ? std::to_string((long long)rate).c_str() : "\"saturation\"", nreq / secs,
+                  pct(lat, 0.5), pct(lat, 0.99), mean(lat), flops / secs * 1e-9);
+    std::fprintf(stderr, "server %s\n", buf);
+    return std::string(buf);
+  };
+
+  auto run_percall_sync = [&]() {
+    std::vector<double> lat;
+    const double t0 = now_us();
+    for (const auto& q : reqs) {
+      Mat& M = mats[q.mat];
+      const double a = now_us();
+      CHECK(lespmv_spmm(M.h, LESPMV_CSR, LESPMV_VARIANT_AUTO, q.k, 1.0, M.X, q.k, 0.0, M.Y, q.k, nullptr));
+      CHECK(lespmv_synchronize());
+      lat.push_back(now_us() - a);
+    }
+    const double secs = (now_us() - t0) * 1e-6;
+    char buf[384];
+    std::snprintf(buf, sizeof buf, "{\"req_per_s\": %.1f, \"p50_us\": %.1f, \"p99_us\": %.1f, \"gflops\": %.1f}",
+                  nreq / secs, pct(lat, 0.5), pct(lat, 0.99), flops / secs * 1e-9);
+    std::fprintf(stderr, "per-call sync %s\n", buf);
+    return std::string(buf);
+  };
+
+  auto run_percall_async = [&]() {
+    const double t0 = now_us();
+    for (size_t r = 0; r < reqs.size(); r++) {
+      const Req& q = reqs[r];
+      Mat& M = mats[q.mat];
+      CHECK(lespmv_spmm(M.h, LESPMV_CSR, LESPMV_VARIANT_AUTO, q.k, 1.0, M.X, q.k, 0.0, (r & 1) ? M.Y2 : M.Y, q.k,
+                        nullptr));
+    }
+    CHECK(lespmv_synchronize());
+    const double secs = (now_us() - t0) * 1e-6;
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "{\"req_per_s\": %.1f, \"gflops\": %.1f}", nreq / secs, flops / secs * 1e-9);
+    std::fprintf(stderr, "per-call async %s\n", buf);
+    return std::string(buf);
+  };
+
+  // warm-up: every matrix once through both paths
+  for (int i = 0; i < nmat; i++) {
+    CHECK(lespmv_server_submit(S, mats[i].id, 4, 1.0, mats[i].X, 4, 0.0, mats[i].Y, 4, nullptr));
+    CHECK(lespmv_spmm(mats[i].h, LESPMV_CSR, LESPMV_VARIANT_AUTO, 4, 1.0, mats[i].X, 4, 0.0, mats[i].Y2, 4, nullptr));
+  }
+  CHECK(lespmv_server_drain(S));
+  CHECK(lespmv_synchronize());
+  {
+    int32_t n = 0;
+    std::vector<int64_t> ids(nmat + 16);
+    CHECK(lespmv_server_poll(S, 0, ids.data(), nullptr, (int32_t)ids.size(), &n));
+  }
+
+  const std::string sat = run_server(0.0);
+  std::vector<std::string> loads;
+  for (double r : rates) loads.push_back(run_server(r));
+  const std::string sync = run_percall_sync();
+  const std::string async = run_percall_async();
+  CHECK(lespmv_server_stats(S, &st));
+
+  std::printf("{\"matrices\": %d, \"requests\": %d, \"total_rows\": %lld, \"total_nnz\": %lld, "
+              "\"resident_bytes\": %lld, \"build_s\": %.1f, \"batches\": %lld, \"server_saturation\": %s, "
+              "\"server_poisson\": [",
+              nmat, nreq, (long long)total_rows, (long long)total_nnz, (long long)st.bytes_used, build_s,
+              (long long)st.batches, sat.c_str());
+  for (size_t i = 0; i < loads.size(); i++) std::printf("%s%s", i ? ", " : "", loads[i].c_str());
+  std::printf("], \"percall_sync\": %s, \"percall_async\": %s}\n", sync.c_str(), async.c_str());
+  for (auto& M : mats) {
+    lespmv_free(M.X);
+    lespmv_free(M.Y);
+    lespmv_free(M.Y2);
+    lespmv_destroy(M.h);
+  }
+  lespmv_server_destroy(S);
+  return 0;
+}

@@ -743,6 +743,25 @@ void compute(Matrix& A, int32_t flags, lespmv_features_t& F, lespmv_feature_imag
   F.device_ms = ms;
 }
 
+// ---- block occupancy (selector input, no reference counterpart) -------------
+
+__global__ void blk_keys(int64_t m, int br, int bc, u64 nb, const int32_t* __restrict__ rp,
+                         const int32_t* __restrict__ ci, u64* __restrict__ keys) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < m) {
+    const u64 row_key = (u64)(r / br) * nb;
+    for (int j = rp[r]; j < rp[r + 1]; j++) keys[j] = row_key + (u64)(ci[j] / bc);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void count_unique(const u64* __restrict__ keys, int64_t n, u64* __restrict__ out) {
+  u64 t = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    t += (i == 0 || keys[i] != keys[i - 1]);
+  t = wave_sum(t);
+  if ((threadIdx.x & 63) == 0 && t) atomicAdd(out, t);
+}
+
 // .features line names in file order (src/sparse_features.cpp:1347-1400).
 const char* const kNames[LESPMV_NUM_FEATURES] = {
     "num_rows", "num_cols", "num_nnz", "nnz_ratio", "nnz_on_maindiag_ratio", "nnz_row_distance_ratio",
@@ -783,6 +802,35 @@ extern "C" int lespmv_features_vector(const lespmv_features_t* f, double* o) {
         (double)f->t_max_nnz_each_CB, f->t_ave_nnz_CB, f->t_standard_dev_CB, f->t_P_ratio_CB, f->t_Gini_CB,
         f->uniqR, f->uniqC, f->potReuseR, f->potReuseC};
     std::memcpy(o, v, sizeof v);
+  });
+}
+
+extern "C" int lespmv_block_counts(lespmv_matrix_t h, const int32_t* dims, int32_t n, int64_t* nnzb) {
+  return lsp::guarded_call([&] {
+    LSP_REQUIRE(h && (n == 0 || (dims && nnzb)), LESPMV_ERR_ARG, "null argument");
+    Matrix& A = *reinterpret_cast<Matrix*>(h);
+    hipStream_t s = A.stream;
+    DBuf keys, out;
+    keys.alloc(sizeof(u64) * std::max<int64_t>(A.nnz, 1));
+    out.alloc(sizeof(u64));
+    for (int32_t i = 0; i < n; i++) {
+      const int br = dims[2 * i], bc = dims[2 * i + 1];
+      LSP_REQUIRE(br > 0 && bc > 0, LESPMV_ERR_ARG, "block dimensions must be positive");
+      const u64 mb = (u64)ceil_div(A.m, br), nb = (u64)ceil_div(A.n, bc);
+      if (A.nnz == 0) {
+        nnzb[i] = 0;
+        continue;
+      }
+      hipLaunchKernelGGL(blk_keys, grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, br, bc, nb, A.rp.as<int32_t>(),
+                         A.ci.as<int32_t>(), keys.as<u64>());
+      LSP_LAUNCHED();
+      radix_sort_u64(keys.as<uint64_t>(), A.nnz, bits_for(mb * nb), s);
+      LSP_HIP(hipMemsetAsync(out.p, 0, sizeof(u64), s));
+      hipLaunchKernelGGL(count_unique, dim3((unsigned)std::min<int64_t>(ceil_div(A.nnz, kBlock), 2048)),
+                         dim3(kBlock), 0, s, keys.as<u64>(), A.nnz, out.as<u64>());
+      LSP_LAUNCHED();
+      nnzb[i] = (int64_t)to_host<u64>(out.p, 1, s)[0];
+    }
   });
 }
 

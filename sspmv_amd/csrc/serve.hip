@@ -1,0 +1,436 @@
+// Batched SpMV / SpMM serving: a packed persistent kernel replayed from a HIP
+// graph, and a continuous-batching request loop with admission of resident
+// matrices by HBM footprint. No reference counterpart (the reference runs one
+// matrix per process, test/benchmark_spmv_<fmt>.cpp).
+//
+// Resident matrices are the server's own CSR copies with a row-chunk table
+// (chunks of <= kChunkNnz nonzeros, <= kChunkRows rows) and a lanes-per-row
+// value from the autotuner. A batch is a table of request descriptors (matrix
+// arrays, X, Y, k, alpha, beta, first work item); its work items are (request,
+// chunk) pairs numbered consecutively. The persistent kernel's waves take items
+// from an atomic counter, find the request by a binary search over the first
+// item numbers held in LDS, and compute the chunk's rows for all k right-hand
+// sides: a row gets G lanes (G = max(lanes, k rounded up to a power of two)),
+// lane t handles right-hand side t mod kk and every (G/kk)-th nonzero, and the
+// partial sums meet in xor shuffles. The lanes a row gets depend only on its
+// matrix and k, never on the batch, so a request's result is bitwise the same
+// alone and in any mix.
+//
+// Each in-flight slot owns a stream, pinned host table, device table, counter
+// and a captured graph: copy the table in, reset the counter with a one-thread
+// kernel (no memset node), run the persistent kernel.
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <vector>
+
+#include "common.hpp"
+
+namespace lsp {
+
+namespace {
+
+constexpr int kMaxBatch = 256;
+constexpr int kChunkNnz = 2048;
+constexpr int kChunkRows = 512;
+constexpr int kServeBlock = 256;
+
+struct alignas(16) ReqDesc {
+  const int* rp;
+  const int* ci;
+  const void* val;
+  const int* chunk_row;  // [nchunks + 1] first row of each chunk
+  const void* X;
+  void* Y;
+  int64_t ldx, ldy;
+  double alpha, beta;
+  int32_t k, lanes;
+  int32_t item_begin, nchunks;
+};
+
+struct alignas(16) BatchHeader {
+  int32_t nreq, total_items, pad0, pad1;
+};
+
+constexpr size_t kTableBytes = sizeof(BatchHeader) + sizeof(ReqDesc) * kMaxBatch;
+
+__global__ void reset_counter(int* c) {
+  if (threadIdx.x == 0) *c = 0;
+}
+
+template <class V>
+__device__ __forceinline__ void run_rows(const ReqDesc& q, int r0, int r1, int lane) {
+  const int k = q.k;
+  int kk = 1;
+  while (kk < k) kk <<= 1;
+  const int G = q.lanes > kk ? q.lanes : kk;  // lanes per row (power of two <= 64)
+  const int P = 64 / G;                       // rows per pass
+  const int grp = lane / G, t = lane % G, j = t % kk, s = t / kk, nzl = G / kk;
+  const V* __restrict__ val = static_cast<const V*>(q.val);
+  const V* __restrict__ X = static_cast<const V*>(q.X);
+  V* __restrict__ Y = static_cast<V*>(q.Y);
+  const V alpha = (V)q.alpha, beta = (V)q.beta;
+  for (int base = r0; base < r1; base += P) {
+    const int row = base + grp;
+    const bool ok = row < r1 && j < k;
+    V acc = 0;
+    if (ok) {
+      const int e = q.rp[row + 1];
+      for (int p = q.rp[row] + s; p < e; p += nzl) acc += val[p] * X[(int64_t)q.ci[p] * q.ldx + j];
+    }
+    for (int off = kk; off < G; off <<= 1) acc += __shfl_xor(acc, off);
+    if (ok && s == 0) {
+      V* y = Y + (int64_t)row * q.ldy + j;
+      *y = beta == (V)0 ? alpha * acc : alpha * acc + beta * *y;
+    }
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(kServeBlock) void packed_kernel(const unsigned char* __restrict__ table,
+                                                              int* __restrict__ counter) {
+  __shared__ int s_begin[kMaxBatch + 1];
+  const BatchHeader* H = reinterpret_cast<const BatchHeader*>(table);
+  const ReqDesc* R = reinterpret_cast<const ReqDesc*>(table + sizeof(BatchHeader));
+  const int nreq = H->nreq, total = H->total_items;
+  for (int i = threadIdx.x; i < nreq; i += kServeBlock) s_begin[i] = R[i].item_begin;
+  if (threadIdx.x == 0) s_begin[nreq] = total;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(counter, 1);
+    item = __shfl(item, 0);
+    if (item >= total) break;  // every wave ends here once the items are taken
+    int lo = 0, hi = nreq - 1;  // last request whose first item <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_begin[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const ReqDesc& q = R[lo];
+    const int c = item - s_begin[lo];
+    run_rows<V>(q, q.chunk_row[c], q.chunk_row[c + 1], lane);
+  }
+}
+
+double now_us() {
+  return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct ServerMat {
+  std::unique_ptr<Matrix> A;
+  lespmv_matrix_t h = nullptr;  // the same object, for the C API
+  DBuf chunks;
+  int32_t nchunks = 0, lanes = 1;
+  int64_t bytes = 0, last_use = 0, pending = 0;
+  bool live = false;
+};
+
+struct Pending {
+  int64_t id;
+  int32_t mat, k;
+  double alpha, beta;
+  const void* X;
+  int64_t ldx;
+  void* Y;
+  int64_t ldy;
+  double t_submit;
+};
+
+struct Slot {
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  unsigned char* h_table = nullptr;
+  DBuf d_table, counter;
+  std::vector<Pending> reqs;
+  bool busy = false;
+};
+
+struct Server {
+  int dtype = LESPMV_F64;
+  int64_t budget = 0, used = 0;
+  int32_t max_batch = kMaxBatch;
+  int grid = 0;
+  std::vector<ServerMat> mats;
+  std::deque<Pending> queue;
+  std::vector<Slot> slots;
+  int64_t next_id = 0, tick = 0;
+  int64_t batches = 0, items = 0, done_count = 0;
+  std::vector<std::pair<int64_t, double>> finished;  // (id, latency us) not yet returned
+
+  ~Server() {
+    for (auto& sl : slots) {
+      if (sl.s) (void)hipStreamSynchronize(sl.s);
+      if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+      if (sl.graph) (void)hipGraphDestroy(sl.graph);
+      if (sl.done) (void)hipEventDestroy(sl.done);
+      if (sl.h_table) (void)hipHostFree(sl.h_table);
+      if (sl.s) (void)hipStreamDestroy(sl.s);
+    }
+  }
+
+  void build_slot(Slot& sl) {
+    LSP_HIP(hipStreamCreate(&sl.s));  // default priority
+    LSP_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    LSP_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_table), kTableBytes, hipHostMallocDefault));
+    std::memset(sl.h_table, 0, kTableBytes);
+    sl.d_table.alloc(kTableBytes);
+    sl.counter.alloc(sizeof(int));
+    LSP_HIP(hipStreamBeginCapture(sl.s, hipStreamCaptureModeThreadLocal));
+    LSP_HIP(hipMemcpyAsync(sl.d_table.p, sl.h_table, kTableBytes, hipMemcpyHostToDevice, sl.s));
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, sl.s, sl.counter.as<int>());
+    if (dtype == LESPMV_F64)
+      hipLaunchKernelGGL(packed_kernel<double>, dim3(grid), dim3(kServeBlock), 0, sl.s,
+                         sl.d_table.as<unsigned char>(), sl.counter.as<int>());
+    else
+      hipLaunchKernelGGL(packed_kernel<float>, dim3(grid), dim3(kServeBlock), 0, sl.s,
+                         sl.d_table.as<unsigned char>(), sl.counter.as<int>());
+    LSP_HIP(hipStreamEndCapture(sl.s, &sl.graph));
+    LSP_HIP(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
+  }
+
+  // evict idle matrices, least recently used first, until `need` bytes fit
+  bool make_room(int64_t need) {
+    while (used + need > budget) {
+      int victim = -1;
+      for (int i = 0; i < (int)mats.size(); i++)
+        if (mats[i].live && mats[i].pending == 0 && (victim < 0 || mats[i].last_use < mats[victim].last_use))
+          victim = i;
+      if (victim < 0) return false;
+      evict(victim);
+    }
+    return true;
+  }
+
+  void evict(int i) {
+    ServerMat& sm = mats[i];
+    used -= sm.bytes;
+    sm.A.reset();
+    sm.h = nullptr;
+    sm.chunks.reset();
+    sm.live = false;
+  }
+
+  void launch(Slot& sl) {
+    const int n = (int)std::min<size_t>(queue.size(), (size_t)max_batch);
+    sl.reqs.assign(queue.begin(), queue.begin() + n);
+    queue.erase(queue.begin(), queue.begin() + n);
+    auto* H = reinterpret_cast<BatchHeader*>(sl.h_table);
+    auto* R = reinterpret_cast<ReqDesc*>(sl.h_table + sizeof(BatchHeader));
+    int32_t item = 0;
+    for (int i = 0; i < n; i++) {
+      const Pending& p = sl.reqs[i];
+      ServerMat& sm = mats[p.mat];
+      Matrix& A = *sm.A;
+      R[i] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, sm.chunks.as<int>(), p.X, p.Y, p.ldx, p.ldy,
+                     p.alpha, p.beta, p.k, sm.lanes, item, sm.nchunks};
+      item += sm.nchunks;
+      sm.last_use = ++tick;
+    }
+    H->nreq = n;
+    H->total_items = item;
+    LSP_HIP(hipGraphLaunch(sl.exec, sl.s));
+    LSP_HIP(hipEventRecord(sl.done, sl.s));
+    sl.busy = true;
+    batches++;
+    items += item;
+  }
+
+  // retire finished slots; launch queued requests on free slots
+  void progress(bool wait) {
+    for (;;) {
+      bool retired = false;
+      for (auto& sl : slots) {
+        if (!sl.busy) continue;
+        const hipError_t q = hipEventQuery(sl.done);
+        if (q == hipErrorNotReady) continue;
+        LSP_HIP(q);
+        const double t = now_us();
+        for (const auto& p : sl.reqs) {
+          finished.emplace_back(p.id, t - p.t_submit);
+          mats[p.mat].pending--;
+        }
+        done_count += (int64_t)sl.reqs.size();
+        sl.reqs.clear();
+        sl.busy = false;
+        retired = true;
+      }
+      for (auto& sl : slots)
+        if (!sl.busy && !queue.empty()) launch(sl);
+      const bool any_busy = std::any_of(slots.begin(), slots.end(), [](const Slot& s) { return s.busy; });
+      if (!wait || retired || !finished.empty() || !any_busy) return;
+    }
+  }
+};
+
+std::vector<int32_t> row_chunks(const std::vector<int32_t>& rp) {
+  std::vector<int32_t> starts{0};
+  const int64_t m = (int64_t)rp.size() - 1;
+  int64_t r = 0;
+  while (r < m) {
+    int64_t e = r + 1;
+    while (e < m && e - r < kChunkRows && rp[e + 1] - rp[r] <= kChunkNnz) e++;
+    starts.push_back((int32_t)e);
+    r = e;
+  }
+  return starts;
+}
+
+}  // namespace lsp
+
+using namespace lsp;
+
+namespace {
+Server& srv(lespmv_server_t h) {
+  LSP_REQUIRE(h, LESPMV_ERR_ARG, "null server");
+  return *reinterpret_cast<Server*>(h);
+}
+}  // namespace
+
+extern "C" int lespmv_server_create(lespmv_server_t* out, int32_t dtype, int64_t hbm_budget, int32_t max_batch,
+                                    int32_t nslots) {
+  return guarded_call([&] {
+    LSP_REQUIRE(out && (dtype == LESPMV_F32 || dtype == LESPMV_F64) && hbm_budget > 0 && max_batch >= 1 &&
+                    max_batch <= kMaxBatch && nslots >= 1 && nslots <= 8,
+                LESPMV_ERR_ARG, "bad server arguments");
+    auto S = std::make_unique<Server>();
+    S->dtype = dtype;
+    S->budget = hbm_budget;
+    S->max_batch = max_batch;
+    int dev = 0;
+    LSP_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    LSP_HIP(hipGetDeviceProperties(&p, dev));
+    S->grid = p.multiProcessorCount * 2;  // 8 waves per CU, persistent
+    S->slots.resize(nslots);
+    for (auto& sl : S->slots) S->build_slot(sl);
+    *out = reinterpret_cast<lespmv_server_t>(S.release());
+  });
+}
+
+extern "C" int lespmv_server_destroy(lespmv_server_t h) {
+  return guarded_call([&] { delete reinterpret_cast<Server*>(h); });
+}
+
+extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, int64_t nnz, const int32_t* rp,
+                                     const int32_t* ci, const void* val, int32_t lanes, int32_t* id) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(id && rp && (nnz == 0 || (ci && val)) && m > 0 && n > 0, LESPMV_ERR_ARG, "bad matrix arguments");
+    LSP_REQUIRE(lanes == 0 || (lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0), LESPMV_ERR_ARG,
+                "lanes must be 0 (tune) or a power of two <= 64");
+    const std::vector<int32_t> hrp(rp, rp + m + 1);
+    const std::vector<int32_t> starts = row_chunks(hrp);
+    const int64_t vs = vsize(S.dtype);
+    const int64_t bytes = (m + 1) * 4 + nnz * (4 + vs) + (int64_t)starts.size() * 4;
+    LSP_REQUIRE(bytes <= S.budget, LESPMV_ERR_OOM, "matrix larger than the server's HBM budget");
+    LSP_REQUIRE(S.make_room(bytes), LESPMV_ERR_OOM, "no room: every resident matrix has requests in flight");
+    lespmv_matrix_t mh = nullptr;
+    int rc = lespmv_create_csr(&mh, S.dtype, m, n, nnz, rp, ci, val, 0);
+    LSP_REQUIRE(rc == LESPMV_OK, rc, "creating the resident matrix failed");
+    ServerMat sm;
+    sm.A.reset(reinterpret_cast<Matrix*>(mh));
+    sm.h = mh;
+    sm.chunks.alloc(sizeof(int32_t) * starts.size());
+    LSP_HIP(hipMemcpy(sm.chunks.p, starts.data(), sizeof(int32_t) * starts.size(), hipMemcpyHostToDevice));
+    sm.nchunks = (int32_t)starts.size() - 1;
+    if (lanes == 0) {  // the autotuner's lanes per row (rows per wave = 64 / lanes) for this matrix
+      int32_t count = 0;
+      lespmv_tune_candidates(1 << LESPMV_CSR, S.dtype, nullptr, 0, &count);
+      std::vector<lespmv_tune_result_t> cand(count);
+      lespmv_tune_candidates(1 << LESPMV_CSR, S.dtype, cand.data(), count, &count);
+      rc = lespmv_tune(mh, cand.data(), count, 5, 1);
+      lanes = 8;
+      double best = 1e300;
+      for (const auto& c : cand)
+        if (rc == LESPMV_OK && c.status == LESPMV_OK && c.variant >= 17 && c.variant <= 22 && c.ms < best) {
+          best = c.ms;
+          lanes = 1 << (c.variant - 16);
+        }
+    }
+    sm.lanes = lanes;
+    sm.bytes = bytes;
+    sm.last_use = ++S.tick;
+    sm.live = true;
+    S.used += bytes;
+    // ids are never reused, so a stale id of an evicted matrix stays an error
+    S.mats.push_back(std::move(sm));
+    *id = (int32_t)S.mats.size() - 1;
+  });
+}
+
+extern "C" int lespmv_server_remove(lespmv_server_t h, int32_t id) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(id >= 0 && id < (int)S.mats.size() && S.mats[id].live, LESPMV_ERR_ARG, "matrix not resident");
+    LSP_REQUIRE(S.mats[id].pending == 0, LESPMV_ERR_ARG, "matrix has requests in flight");
+    S.evict(id);
+  });
+}
+
+extern "C" int lespmv_server_matrix(lespmv_server_t h, int32_t id, lespmv_matrix_t* A, int32_t* lanes) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(id >= 0 && id < (int)S.mats.size() && S.mats[id].live, LESPMV_ERR_ARG, "matrix not resident");
+    if (A) *A = S.mats[id].h;
+    if (lanes) *lanes = S.mats[id].lanes;
+  });
+}
+
+extern "C" int lespmv_server_submit(lespmv_server_t h, int32_t id, int32_t k, double alpha, const void* X,
+                                    int64_t ldx, double beta, void* Y, int64_t ldy, int64_t* req) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(id >= 0 && id < (int)S.mats.size() && S.mats[id].live, LESPMV_ERR_ARG, "matrix not resident");
+    LSP_REQUIRE(k >= 1 && k <= 32 && ldx >= k && ldy >= k && X && Y, LESPMV_ERR_ARG, "bad request");
+    Pending p{S.next_id++, id, k, alpha, beta, X, ldx, Y, ldy, now_us()};
+    S.mats[id].pending++;
+    S.queue.push_back(p);
+    if (req) *req = p.id;
+    S.progress(false);
+  });
+}
+
+extern "C" int lespmv_server_poll(lespmv_server_t h, int32_t wait, int64_t* ids, double* latency_us, int32_t cap,
+                                  int32_t* ndone) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(ndone && (cap == 0 || ids), LESPMV_ERR_ARG, "bad poll arguments");
+    S.progress(wait != 0);
+    const int n = (int)std::min<size_t>(S.finished.size(), (size_t)std::max(cap, 0));
+    for (int i = 0; i < n; i++) {
+      ids[i] = S.finished[i].first;
+      if (latency_us) latency_us[i] = S.finished[i].second;
+    }
+    S.finished.erase(S.finished.begin(), S.finished.begin() + n);
+    *ndone = n;
+  });
+}
+
+extern "C" int lespmv_server_drain(lespmv_server_t h) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    while (!S.queue.empty() ||
+           std::any_of(S.slots.begin(), S.slots.end(), [](const Slot& s) { return s.busy; }))
+      S.progress(true);
+  });
+}
+
+extern "C" int lespmv_server_stats(lespmv_server_t h, lespmv_server_stats_t* st) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(st, LESPMV_ERR_ARG, "null stats");
+    st->resident = (int32_t)std::count_if(S.mats.begin(), S.mats.end(), [](const ServerMat& m) { return m.live; });
+    st->queued = (int32_t)S.queue.size();
+    st->bytes_used = S.used;
+    st->bytes_budget = S.budget;
+    st->batches = S.batches;
+    st->items = S.items;
+    st->completed = S.done_count;
+  });
+}

@@ -181,8 +181,9 @@ def csr_to_csr5(csr: CSR_Matrix, fp_feature=None) -> CSR5_Matrix:
 
 # ---- SpMV entry points (include/spmv_*.h) --------------------------------------
 
-def _spmv(A: Matrix_Features, alpha, x, beta, y):
-    """x, y: numpy arrays (staged through the device) or lib.DeviceArray."""
+def _spmv(alpha, A: Matrix_Features, x, beta, y):
+    """LeSpMV_<fmt>(alpha, A, x, beta, y); x, y: numpy arrays (staged through
+    the device) or lib.DeviceArray."""
     M = A.device() if isinstance(A, CSR_Matrix) else A.gpu
     if isinstance(x, lib.DeviceArray):
         M.spmv(A.fmt, alpha, x, beta, y, variant=A.kernel_flag)
@@ -217,7 +218,7 @@ def compare_spmv_kernels(csr_ref: CSR_Matrix, A: Matrix_Features, seed: int = 0)
     ref = CSR_Matrix(csr_ref.num_rows, csr_ref.num_cols, csr_ref.row_offset, csr_ref.col_index, csr_ref.values)
     ref.kernel_flag = 0
     LeSpMV_csr(0.8, ref, x, 0.7, y1)
-    _spmv(A, 0.8, x, 0.7, y2)
+    _spmv(0.8, A, x, 0.7, y2)
     return maximum_relative_error(y1, y2)
 
 

@@ -70,6 +70,20 @@ FEAT_TILES = 256
 NUM_FEATURES = 40
 
 
+class TuneResultC(C.Structure):
+    _fields_ = [("fmt", C.c_int32), ("variant", C.c_int32), ("params", Params), ("status", C.c_int32),
+                ("ms", C.c_double), ("convert_ms", C.c_double)]
+
+    def as_dict(self):
+        return {"fmt": self.fmt, "variant": self.variant, "params": self.params.as_dict(), "status": self.status,
+                "ms": self.ms, "convert_ms": self.convert_ms}
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [("resident", C.c_int32), ("queued", C.c_int32), ("bytes_used", C.c_int64),
+                ("bytes_budget", C.c_int64), ("batches", C.c_int64), ("items", C.c_int64), ("completed", C.c_int64)]
+
+
 class FeaturesC(C.Structure):
     _fields_ = [(n, C.c_int64) for n in FEATURE_INT_FIELDS] + [(n, C.c_double) for n in FEATURE_DOUBLE_FIELDS]
 
@@ -136,6 +150,34 @@ def load() -> C.CDLL:
         "lespmv_read_mtx": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(HostCsrC)], C.c_int),
         "lespmv_coo_to_csr": ([C.c_int, i64, i64, i64, vp, vp, vp, C.c_int, C.POINTER(HostCsrC)], C.c_int),
         "lespmv_features": ([vp, i32, C.POINTER(FeaturesC), C.POINTER(FeatureImagesC)], C.c_int),
+        "lespmv_tune_candidates": ([i32, i32, C.POINTER(TuneResultC), i32, C.POINTER(i32)], C.c_int),
+        "lespmv_tune": ([vp, C.POINTER(TuneResultC), i32, i32, i32], C.c_int),
+        "lespmv_autotune": ([vp, i32, i32, i32, C.POINTER(TuneResultC)], C.c_int),
+        "lespmv_block_counts": ([vp, vp, i32, vp], C.c_int),
+        "lespmv_mlp_create": ([C.POINTER(vp), i32, i32, i32, i32, vp, C.c_uint64], C.c_int),
+        "lespmv_mlp_destroy": ([vp], C.c_int),
+        "lespmv_mlp_shape": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)], C.c_int),
+        "lespmv_mlp_get_params": ([vp, vp], C.c_int),
+        "lespmv_mlp_set_params": ([vp, vp], C.c_int),
+        "lespmv_mlp_set_normalizer": ([vp, vp, vp], C.c_int),
+        "lespmv_mlp_loss_grad": ([vp, vp, vp, i32, C.POINTER(dbl), vp], C.c_int),
+        "lespmv_mlp_train": ([vp, vp, vp, i32, i32, i32, C.c_float, C.c_float, C.c_uint64, vp], C.c_int),
+        "lespmv_mlp_forward": ([vp, vp, i32, vp], C.c_int),
+        "lespmv_mlp_save": ([vp, C.c_char_p], C.c_int),
+        "lespmv_mlp_load": ([C.POINTER(vp), C.c_char_p], C.c_int),
+        "lespmv_selector_layout": ([vp, vp, vp], C.c_int),
+        "lespmv_selector_inputs": ([vp, i32, vp, vp, C.POINTER(FeaturesC)], C.c_int),
+        "lespmv_selector_decode": ([vp, vp, C.POINTER(i32), C.POINTER(i32)], C.c_int),
+        "lespmv_select": ([vp, vp, i32, C.POINTER(TuneResultC), C.POINTER(i32)], C.c_int),
+        "lespmv_server_create": ([C.POINTER(vp), i32, i64, i32, i32], C.c_int),
+        "lespmv_server_destroy": ([vp], C.c_int),
+        "lespmv_server_add_csr": ([vp, i64, i64, i64, vp, vp, vp, i32, C.POINTER(i32)], C.c_int),
+        "lespmv_server_remove": ([vp, i32], C.c_int),
+        "lespmv_server_matrix": ([vp, i32, C.POINTER(vp), C.POINTER(i32)], C.c_int),
+        "lespmv_server_submit": ([vp, i32, i32, dbl, vp, i64, dbl, vp, i64, C.POINTER(i64)], C.c_int),
+        "lespmv_server_poll": ([vp, i32, vp, vp, i32, C.POINTER(i32)], C.c_int),
+        "lespmv_server_drain": ([vp], C.c_int),
+        "lespmv_server_stats": ([vp, C.POINTER(ServerStatsC)], C.c_int),
         "lespmv_features_vector": ([C.POINTER(FeaturesC), C.POINTER(dbl)], C.c_int),
         "lespmv_feature_name": ([i32], C.c_char_p),
     }
@@ -374,8 +416,206 @@ class Matrix:
     def features(self, symmetric: bool = False, images: bool = False):
         return features(self, symmetric, images)
 
+    def tune(self, candidates=None, iters: int = 10, k: int = 1):
+        """Time every candidate (converted, timed and released in turn)."""
+        if candidates is None:
+            candidates = tune_candidates(ALL_FORMATS_MASK, self.dtype)
+        check(load().lespmv_tune(self.h, candidates, len(candidates), iters, k))
+        return candidates
+
+    def autotune(self, fmt: int, iters: int = 10, k: int = 1) -> TuneResultC:
+        """Best parameters of one format; the winner stays converted."""
+        best = TuneResultC()
+        check(load().lespmv_autotune(self.h, fmt, iters, k, C.byref(best)))
+        return best
+
     def export_scalar(self, fmt: int, name: str) -> int:
         return int(self.export(fmt, name, np.int64)[0])
+
+
+ALL_FORMATS_MASK = (1 << 9) - 1
+
+
+def tune_candidates(fmt_mask: int = ALL_FORMATS_MASK, dtype=np.float64):
+    """The library's standard candidate grid (a ctypes array of TuneResultC)."""
+    lib = load()
+    n = C.c_int32()
+    check(lib.lespmv_tune_candidates(fmt_mask, dtype_id(dtype), None, 0, C.byref(n)))
+    arr = (TuneResultC * n.value)()
+    check(lib.lespmv_tune_candidates(fmt_mask, dtype_id(dtype), arr, n.value, C.byref(n)))
+    return arr
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
+class Mlp:
+    """The GPU-trained MLP (lespmv_mlp_*): in -> h1 -> h2 -> heads."""
+
+    def __init__(self, in_dim=None, h1=128, h2=128, head_sizes=(), seed=0, path=None):
+        lib = load()
+        self.h = C.c_void_p()
+        if path is not None:
+            check(lib.lespmv_mlp_load(C.byref(self.h), path.encode()))
+        else:
+            hs, hp = _i32(head_sizes)
+            check(lib.lespmv_mlp_create(C.byref(self.h), in_dim, h1, h2, len(hs), hp, seed))
+        i, o, n = C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib.lespmv_mlp_shape(self.h, C.byref(i), C.byref(o), C.byref(n)))
+        self.in_dim, self.out_dim, self.nparams = i.value, o.value, n.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().lespmv_mlp_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparams, dtype=np.float32)
+        check(load().lespmv_mlp_get_params(self.h, p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def set_params(self, p):
+        p, pp = _f32(p)
+        check(load().lespmv_mlp_set_params(self.h, pp))
+
+    def set_normalizer(self, mean, std):
+        m, mp = _f32(mean)
+        s, sp = _f32(std)
+        check(load().lespmv_mlp_set_normalizer(self.h, mp, sp))
+
+    def loss_grad(self, X, labels):
+        X, xp = _f32(X)
+        L, lp = _i32(labels)
+        loss = C.c_double()
+        g = np.zeros(self.nparams, dtype=np.float32)
+        check(load().lespmv_mlp_loss_grad(self.h, xp, lp, X.shape[0], C.byref(loss), g.ctypes.data_as(C.c_void_p)))
+        return loss.value, g
+
+    def train(self, X, labels, epochs, batch=256, lr=1e-3, weight_decay=0.0, seed=0):
+        X, xp = _f32(X)
+        L, lp = _i32(labels)
+        hist = np.zeros(epochs, dtype=np.float32)
+        check(load().lespmv_mlp_train(self.h, xp, lp, X.shape[0], epochs, batch, lr, weight_decay, seed,
+                                      hist.ctypes.data_as(C.c_void_p)))
+        return hist
+
+    def forward(self, X) -> np.ndarray:
+        X, xp = _f32(X)
+        out = np.zeros((X.shape[0], self.out_dim), dtype=np.float32)
+        check(load().lespmv_mlp_forward(self.h, xp, X.shape[0], out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def save(self, path: str):
+        check(load().lespmv_mlp_save(self.h, path.encode()))
+
+
+def selector_layout():
+    cf = np.zeros(9, dtype=np.int32)
+    hc = np.zeros(8, dtype=np.int32)
+    hs = np.zeros(6, dtype=np.int32)
+    check(load().lespmv_selector_layout(cf.ctypes.data_as(C.c_void_p), hc.ctypes.data_as(C.c_void_p),
+                                        hs.ctypes.data_as(C.c_void_p)))
+    return cf, hc, hs
+
+
+def selector_inputs(M: "Matrix", symmetric: bool = False):
+    """(128 float inputs, 32 allowed flags, features dict) of a handle."""
+    x = np.zeros(128, dtype=np.float32)
+    allowed = np.zeros(32, dtype=np.uint8)
+    f = FeaturesC()
+    check(load().lespmv_selector_inputs(M.h, FEAT_SYMMETRIC_FILE if symmetric else 0, x.ctypes.data_as(C.c_void_p),
+                                        allowed.ctypes.data_as(C.c_void_p), C.byref(f)))
+    return x, allowed, {n: getattr(f, n) for n, _ in FeaturesC._fields_}
+
+
+def selector_decode(probs, allowed):
+    p, pp = _f32(probs)
+    a = np.ascontiguousarray(allowed, dtype=np.uint8)
+    cls, cand = C.c_int32(), C.c_int32()
+    check(load().lespmv_selector_decode(pp, a.ctypes.data_as(C.c_void_p), C.byref(cls), C.byref(cand)))
+    return cls.value, cand.value
+
+
+def select(M: "Matrix", mlp: Mlp, symmetric: bool = False):
+    """(chosen TuneResultC, format class) for a handle."""
+    choice = TuneResultC()
+    cls = C.c_int32()
+    check(load().lespmv_select(M.h, mlp.h, FEAT_SYMMETRIC_FILE if symmetric else 0, C.byref(choice), C.byref(cls)))
+    return choice, cls.value
+
+
+def block_counts(M: "Matrix", dims) -> np.ndarray:
+    d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1))
+    out = np.zeros(len(d) // 2, dtype=np.int64)
+    check(load().lespmv_block_counts(M.h, d.ctypes.data_as(C.c_void_p), len(out), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+class Server:
+    """Continuous-batching SpMV / SpMM server (lespmv_server_*)."""
+
+    def __init__(self, dtype=np.float64, budget_bytes: int = 1 << 34, max_batch: int = 128, slots: int = 2):
+        self.h = C.c_void_p()
+        self.dtype = np.dtype(dtype)
+        check(load().lespmv_server_create(C.byref(self.h), dtype_id(dtype), int(budget_bytes), max_batch, slots))
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().lespmv_server_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+    def add(self, A, lanes: int = 0) -> int:
+        rp = np.ascontiguousarray(A.rp, dtype=np.int32)
+        ci = np.ascontiguousarray(A.ci, dtype=np.int32)
+        v = np.ascontiguousarray(A.v, dtype=self.dtype)
+        mid = C.c_int32()
+        check(load().lespmv_server_add_csr(self.h, A.m, A.n, len(ci), rp.ctypes.data_as(C.c_void_p),
+                                           ci.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), lanes,
+                                           C.byref(mid)))
+        return mid.value
+
+    def remove(self, mid: int):
+        check(load().lespmv_server_remove(self.h, mid))
+
+    def lanes(self, mid: int) -> int:
+        a, lanes = C.c_void_p(), C.c_int32()
+        check(load().lespmv_server_matrix(self.h, mid, C.byref(a), C.byref(lanes)))
+        return lanes.value
+
+    def submit(self, mid: int, k: int, alpha: float, X: "DeviceArray", ldx: int, beta: float, Y: "DeviceArray",
+               ldy: int) -> int:
+        rid = C.c_int64()
+        check(load().lespmv_server_submit(self.h, mid, k, alpha, X.ptr, ldx, beta, Y.ptr, ldy, C.byref(rid)))
+        return rid.value
+
+    def poll(self, wait: bool = False, cap: int = 4096):
+        ids = np.zeros(cap, dtype=np.int64)
+        lat = np.zeros(cap, dtype=np.float64)
+        n = C.c_int32()
+        check(load().lespmv_server_poll(self.h, int(wait), ids.ctypes.data_as(C.c_void_p),
+                                        lat.ctypes.data_as(C.c_void_p), cap, C.byref(n)))
+        return ids[:n.value], lat[:n.value]
+
+    def drain(self):
+        check(load().lespmv_server_drain(self.h))
+
+    def stats(self) -> dict:
+        s = ServerStatsC()
+        check(load().lespmv_server_stats(self.h, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in ServerStatsC._fields_}
 
 
 def feature_names() -> list:

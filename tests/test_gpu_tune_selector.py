@@ -1,0 +1,107 @@
+"""Exhaustive timing, the autotuner and the learned selector on the GPU."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.conftest import TOL, rel2
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODEL = os.path.join(ROOT, "sspmv_amd", "selector_data", "selector_mi355x.bin")
+DB = os.path.join(ROOT, "sspmv_amd", "selector_data", "timings_mi355x.npz")
+
+
+def test_tune_grid_and_autotune(gpu, oracle):
+    lib = gpu
+    A = lib.generate(lib.GEN_LAPLACE2D, [200, 150, 4.0, -1.0], 0)
+    M = lib.Matrix.from_host(A)
+    cands = M.tune(iters=3)
+    assert len(cands) == 27
+    ok = {(c.fmt, c.variant) for c in cands if c.status == 0}
+    for fmt in (lib.CSR, lib.COO, lib.ELL, lib.SELL, lib.SELL_C_SIGMA, lib.SELL_C_R, lib.DIA, lib.BSR, lib.CSR5):
+        assert any(f == fmt for f, _ in ok), fmt
+    assert all(np.isfinite(c.ms) and c.ms > 0 for c in cands if c.status == 0)
+    assert not any(M.has(f) for f in range(9) if f != lib.CSR)  # everything released
+    best = M.autotune(lib.SELL_C_SIGMA, iters=3)
+    sell = [c.ms for c in cands if c.fmt == lib.SELL_C_SIGMA and c.status == 0]
+    assert best.status == 0 and best.params.chunk in (32, 64) and M.has(lib.SELL_C_SIGMA)
+    assert best.ms <= 1.5 * min(sell)  # same grid, timed again
+    x = lib.fill_uniform(A.n, 9)
+    y = lib.DeviceArray.empty(A.m, np.float64)
+    M.spmv(lib.SELL_C_SIGMA, 1.0, lib.DeviceArray.from_numpy(x), 0.0, y)
+    ref = oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v, 1.0, x, 0.0, np.zeros(A.m))
+    assert rel2(y.numpy(), ref) <= TOL[np.dtype(np.float64)]
+
+
+def test_fill_limit_refuses_padding(gpu):
+    lib = gpu
+    # one long row: ELL would pad every row to 3000
+    rp = np.zeros(3001, dtype=np.int32)
+    rp[1:] = 1
+    rp[1] = 3000
+    rp[2:] = 3000 + np.arange(1, 3000)
+    ci = np.concatenate([np.arange(3000), np.arange(1, 3000)]).astype(np.int32)
+    M = lib.Matrix(3000, 3000, rp, ci, np.ones(len(ci)))
+    with pytest.raises(lib.LespmvError) as e:
+        M.convert(lib.ELL, lib.make_params(max_fill=4))
+    assert e.value.code == 3
+    M.convert(lib.ELL, lib.make_params(max_fill=0))  # no limit
+    assert M.has(lib.ELL)
+
+
+def test_block_counts(gpu):
+    lib = gpu
+    A = lib.generate(lib.GEN_BLOCKRAND, [100, 90, 4, 4, 5], 3)
+    M = lib.Matrix.from_host(A)
+    n2, n4, n8 = lib.block_counts(M, [[2, 2], [4, 4], [8, 8]])
+    assert n4 == 100 * 5  # 5 dense 4x4 blocks per block row
+    assert n2 == 4 * n4
+    rows = np.repeat(np.arange(A.m), np.diff(A.rp))
+    assert n8 == len(np.unique((rows // 8) * 1000 + A.ci // 8))
+
+
+def test_selector_small_pipeline(gpu, tmp_path):
+    """Collect a small timing database, train, and check that lespmv_select
+    decodes the network's output the same way the evaluation does."""
+    from sspmv_amd import selector
+    lib = gpu
+    path = str(tmp_path / "db.npz")
+    db = selector.collect(24, path, seed=5, iters=2, max_nnz=2e5, n_range=(2e3, 2e4), log=lambda *a, **k: None)
+    assert db["X"].shape == (24, 128) and db["times"].shape == (24, 28)
+    assert np.isfinite(db["times"][:, :27].min(axis=1)).all()
+    L = selector.labels(db["times"])
+    assert (L[:, 0] >= 0).all()
+    mlp, hist = selector.train(db["X"], L, epochs=30, seed=1)
+    assert hist[-1] < hist[0]
+    cand = selector.picks(mlp, db["X"], db["allowed"])
+    r = selector.evaluate(db["times"], cand)
+    assert r["mean_regret"] >= 0
+    # the library's own select() on one of the matrices agrees with the decode
+    fam, gen, params = selector.sample_matrix(np.random.default_rng(5), 2e5, (2e3, 2e4))
+    A = lib.generate(gen, params, 5 * 1_000_003)
+    M = lib.Matrix.from_host(A)
+    x, allowed, _ = lib.selector_inputs(M)
+    np.testing.assert_array_equal(x, db["X"][0])
+    choice, cls = lib.select(M, mlp)
+    c2, cand2 = lib.selector_decode(mlp.forward(x[None, :])[0], allowed)
+    grid = lib.tune_candidates()
+    assert cls == c2 and choice.fmt == grid[cand2].fmt and choice.variant == grid[cand2].variant
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained selector committed")
+def test_trained_selector_beats_default(gpu):
+    """The committed selector on the held-out 20 % of the committed timing
+    database: it must gain over the library default (CSR with the automatic
+    kernel), i.e. more matrices within 10 % of the exhaustive best and a lower
+    mean regret."""
+    from sspmv_amd import selector
+    lib = gpu
+    db = selector.load_db(DB)
+    _, te = selector.split(len(db["X"]))
+    mlp = lib.Mlp(path=MODEL)
+    cand = selector.picks(mlp, db["X"][te].astype(np.float32), db["allowed"][te])
+    sel = selector.evaluate(db["times"][te], cand)
+    base = selector.evaluate(db["times"][te], np.full(len(te), selector.N_GRID))
+    assert sel["within_10pct"] > base["within_10pct"]
+    assert sel["mean_regret"] < base["mean_regret"]

@@ -1,0 +1,40 @@
+"""Selector bookkeeping on the CPU: labels from timings, regret, sampling."""
+import numpy as np
+
+from sspmv_amd import lib, selector
+
+
+def test_labels_pick_fastest_and_per_class_best():
+    cf, hc, hs = lib.selector_layout()
+    t = np.full((2, 28), np.inf)
+    t[0, cf[4] + 5] = 1.0   # SELL (C=64, S-ELL) fastest
+    t[0, cf[0] + 2] = 2.0   # CSR vector, 8 lanes
+    t[0, cf[6] + 1] = 3.0   # BSR 4x4
+    t[1, cf[5] + 1] = 0.5   # DIA with LDS windows
+    t[1, cf[5]] = 0.7
+    t[:, 27] = 5.0
+    L = selector.labels(t)
+    assert L[0].tolist() == [4, 2, 5, -1, 1, -1]
+    assert L[1].tolist() == [5, -1, -1, 1, -1, -1]
+    assert hs.tolist() == [8, 6, 8, 2, 5, 3] and cf[-1] == selector.N_GRID
+
+
+def test_regret_and_fallback():
+    t = np.array([[1.0] + [2.0] * 26 + [4.0], [np.inf, 3.0] + [6.0] * 25 + [3.0]])
+    r = selector.evaluate(t, np.array([0, 0]))  # second pick does not apply: default CSR (3.0) is used
+    assert r["within_10pct"] == 1.0 and r["mean_regret"] == 0.0
+    r = selector.evaluate(t, np.array([1, 2]))
+    assert np.isclose(r["mean_regret"], (1.0 + 1.0) / 2)
+
+
+def test_sampling_covers_families_and_respects_nnz_cap():
+    rng = np.random.default_rng(0)
+    fams = set()
+    for _ in range(400):
+        fam, gen, params = selector.sample_matrix(rng, 6e6)
+        fams.add(fam)
+        if fam == "uniform":
+            assert params[0] * params[1] * params[2] <= 6e6 * 1.0001
+        if fam == "blockdiag":
+            assert params[0] * params[1] <= 6e6 * 1.0001 or params[1] == 2
+    assert fams == set(selector.FAMILIES)
