@@ -49,9 +49,10 @@ def kernels(lib, section):
                 ("DIA-lds", lib.DIA, lib.VARIANT_MERGE, lib.make_params(alignment=1, max_diags=10240)),
                 ("BSR-4x4", lib.BSR, 0, lib.make_params(block_rows=4, block_cols=4))]
     ks = [("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None), ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None),
+          ("CSR-panel8", lib.CSR, lib.VARIANT_PANEL, None), ("CSR-panel16", lib.CSR, lib.VARIANT_PANEL16, None),
           ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, sell(64, 4096)), ("SELL-64-R", lib.SELL_C_R, 0, sell(64, 0)),
           ("COO", lib.COO, 0, None), ("CSR5-64x16", lib.CSR5, 0, lib.make_params(omega=64, sigma=16))]
-    return ks if section == "lognormal" else ks[:3]
+    return ks if section == "lognormal" else ks[:5]
 
 
 def traffic_from_profiles(label):
@@ -127,6 +128,31 @@ def serving(matrices, requests, rates):
     return r
 
 
+def spmm_section(lib, steps):
+    """Multi-vector products on a 4x4-block random matrix (1M rows, 8 blocks per
+    block row, the batched workload's block family at full size): BSR 4x4 on
+    the matrix cores, BSR scalar, CSR; GFLOP/s and effective GB/s."""
+    A = lib.generate(lib.GEN_BLOCKRAND, [262144, 262144, 4, 4, 8], 11)
+    M = lib.Matrix.from_host(A)
+    M.convert(lib.BSR, lib.make_params(block_rows=4, block_cols=4))
+    res = {}
+    for k in (4, 16, 32):
+        X = lib.DeviceArray.from_numpy(lib.fill_uniform(A.n * k, 12))
+        Y = lib.DeviceArray.empty(A.m * k, np.float64)
+        row = {}
+        for label, fmt, var in (("BSR-4x4-mfma", lib.BSR, lib.VARIANT_AUTO), ("BSR-4x4-scalar", lib.BSR,
+                                                                              lib.VARIANT_VECTOR),
+                                ("CSR", lib.CSR, lib.VARIANT_AUTO)):
+            M.time_spmm(fmt, k, 1.0, X, k, 0.0, Y, k, 3, var)
+            ms = M.time_spmm(fmt, k, 1.0, X, k, 0.0, Y, k, steps, var)
+            eff = M.effective_bytes(fmt, k, False)
+            row[label] = {"us": round(ms * 1e3, 2), "GFLOPs": round(2.0 * A.nnz * k / (ms * 1e-3) / 1e9, 1),
+                          "GBs": round(eff / ms / 1e6, 1)}
+            log(f"[spmm k={k}] {label}: {ms * 1e3:.1f} us")
+        res[f"k{k}"] = row
+    return res
+
+
 def copy_bandwidth(lib, nbytes=1 << 30, iters=20):
     src = lib.DeviceArray(nbytes)
     dst = lib.DeviceArray(nbytes)
@@ -144,7 +170,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bound on the CPU baseline's timed work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--formats", default="all", help="comma list of kernel labels to measure, or 'all'")
-    ap.add_argument("--sections", default="lognormal,lognormal_fp32,stencil27,copy,serving")
+    ap.add_argument("--sections", default="lognormal,lognormal_fp32,stencil27,spmm,copy,serving")
     ap.add_argument("--serve-matrices", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--serve-rates", default="5000,20000,50000")
@@ -249,6 +275,8 @@ def main():
         dy = lib.DeviceArray.empty(S.m, np.float64)
         out["formats"]["stencil27_f64"] = summary(measure(lib, M, dx, dy, "stencil27", wanted, args.warmup))
         del M, dx, dy, S
+    if "spmm" in sections:
+        out["spmm_blockrand4x4_1M"] = spmm_section(lib, 20)
     if "serving" in sections:
         out["serving"] = serving(args.serve_matrices, args.serve_requests, args.serve_rates)
     print(json.dumps(out), flush=True)

@@ -73,7 +73,10 @@ enum {
   LESPMV_VARIANT_AUTO = 0,      /* the handle's tuned choice */
   LESPMV_VARIANT_VECTOR = 1,    /* CSR: lanes-per-row vector kernel; DIA: x through caches */
   LESPMV_VARIANT_MERGE = 2,     /* CSR: merge-path tiles; DIA: LDS-staged x windows */
-  LESPMV_VARIANT_ATOMIC = 3     /* COO: atomics instead of the carry fix-up pass */
+  LESPMV_VARIANT_ATOMIC = 3,    /* COO: atomics instead of the carry fix-up pass */
+  LESPMV_VARIANT_PANEL = 4,     /* CSR: 8 column panels, one per XCD (x slice held in that XCD's L2),
+                                   merge-path per panel, partial y reduced in panel order */
+  LESPMV_VARIANT_PANEL16 = 5    /* CSR: the same with 16 panels (two per XCD) */
 };
 
 /* Conversion parameters. Fields a format does not use are ignored. */

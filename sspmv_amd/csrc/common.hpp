@@ -167,6 +167,10 @@ struct CsrMeta {
   DBuf carry_val;            // V[tiles]      its partial sum inside the tile
   DBuf head_val;             // V[tiles]      partial of the continued first row
   bool analysed = false;
+  // column panels (CSR panel variants): P column ranges, each its own CSR
+  int32_t panels = 0;
+  int64_t p_maxT = 0;
+  DBuf p_nzoff, p_rp, p_ci, p_val, p_tiles, p_coord, p_carry_row, p_carry_val, p_head, p_partial;
 };
 
 struct Matrix {

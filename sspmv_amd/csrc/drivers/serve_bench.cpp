@@ -245,6 +245,33 @@ int main(int argc, char** argv) {
     CHECK(lespmv_server_poll(S, 0, ids.data(), nullptr, (int32_t)ids.size(), &n));
   }
 
+  // unloaded latency: one request at a time through each path
+  auto run_unloaded = [&](int n) {
+    std::vector<double> ls, lp;
+    std::vector<int64_t> ids(16);
+    std::vector<double> l(16);
+    for (int r = 0; r < n; r++) {
+      const Req& q = reqs[r];
+      Mat& M = mats[q.mat];
+      const double a = now_us();
+      CHECK(lespmv_server_submit(S, M.id, q.k, 1.0, M.X, q.k, 0.0, M.Y, q.k, nullptr));
+      int32_t got = 0;
+      while (got == 0) CHECK(lespmv_server_poll(S, 1, ids.data(), l.data(), 16, &got));
+      ls.push_back(now_us() - a);
+      const double b = now_us();
+      CHECK(lespmv_spmm(M.h, LESPMV_CSR, LESPMV_VARIANT_AUTO, q.k, 1.0, M.X, q.k, 0.0, M.Y2, q.k, nullptr));
+      CHECK(lespmv_synchronize());
+      lp.push_back(now_us() - b);
+    }
+    char buf[384];
+    std::snprintf(buf, sizeof buf,
+                  "{\"server_p50_us\": %.1f, \"server_p99_us\": %.1f, \"percall_p50_us\": %.1f, "
+                  "\"percall_p99_us\": %.1f}",
+                  pct(ls, 0.5), pct(ls, 0.99), pct(lp, 0.5), pct(lp, 0.99));
+    std::fprintf(stderr, "unloaded %s\n", buf);
+    return std::string(buf);
+  };
+  const std::string unloaded = run_unloaded(std::min(nreq, 400));
   const std::string sat = run_server(0.0);
   std::vector<std::string> loads;
   for (double r : rates) loads.push_back(run_server(r));
@@ -258,7 +285,8 @@ int main(int argc, char** argv) {
               nmat, nreq, (long long)total_rows, (long long)total_nnz, (long long)st.bytes_used, build_s,
               (long long)st.batches, sat.c_str());
   for (size_t i = 0; i < loads.size(); i++) std::printf("%s%s", i ? ", " : "", loads[i].c_str());
-  std::printf("], \"percall_sync\": %s, \"percall_async\": %s}\n", sync.c_str(), async.c_str());
+  std::printf("], \"unloaded\": %s, \"percall_sync\": %s, \"percall_async\": %s}\n", unloaded.c_str(),
+              sync.c_str(), async.c_str());
   for (auto& M : mats) {
     lespmv_free(M.X);
     lespmv_free(M.Y);

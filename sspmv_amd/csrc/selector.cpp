@@ -87,6 +87,10 @@ void inputs_from(const lespmv_features_t& f, const int32_t* rb, const int32_t* c
     o[64 + b] = (float)(r / nnz);
     o[96 + b] = (float)(c / nnz);
   }
+  // an empty matrix leaves 0/0 ratios (NaN, as in the reference's features):
+  // the network sees 0 there
+  for (int i = 0; i < LESPMV_SELECTOR_INPUTS; i++)
+    if (!std::isfinite(o[i])) o[i] = 0.f;
   // what can be ruled out from the features (the grid's fill and diagonal limits)
   std::fill(allowed, allowed + kOut, (uint8_t)1);
   const double limit = kMaxFill * nnz + M;

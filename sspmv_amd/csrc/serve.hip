@@ -3,18 +3,19 @@
 // matrices by HBM footprint. No reference counterpart (the reference runs one
 // matrix per process, test/benchmark_spmv_<fmt>.cpp).
 //
-// Resident matrices are the server's own CSR copies with a row-chunk table
-// (chunks of <= kChunkNnz nonzeros, <= kChunkRows rows) and a lanes-per-row
-// value from the autotuner. A batch is a table of request descriptors (matrix
-// arrays, X, Y, k, alpha, beta, first work item); its work items are (request,
-// chunk) pairs numbered consecutively. The persistent kernel's waves take items
-// from an atomic counter, find the request by a binary search over the first
-// item numbers held in LDS, and compute the chunk's rows for all k right-hand
-// sides: a row gets G lanes (G = max(lanes, k rounded up to a power of two)),
-// lane t handles right-hand side t mod kk and every (G/kk)-th nonzero, and the
-// partial sums meet in xor shuffles. The lanes a row gets depend only on its
-// matrix and k, never on the batch, so a request's result is bitwise the same
-// alone and in any mix.
+// Resident matrices are the server's own CSR copies with a lanes-per-row value
+// from the autotuner. A batch is a table of request descriptors (matrix arrays,
+// X, Y, k, alpha, beta, rows per work item, first work item); a request's work
+// items are consecutive row ranges, sized on the host from the matrix's mean
+// row length and k so that an item is a few passes of ~kItemIters loads per
+// lane. The persistent kernel's waves take a few items at a time (the batch's
+// grab size, about eight grabs per wave) from an atomic counter, find the request by a binary search over the first item numbers
+// held in LDS, and compute the rows for all k right-hand sides: a row gets G
+// lanes (G = max(lanes, k rounded up to a power of two)), lane t handles
+// right-hand side t mod kk and every (G/kk)-th nonzero, and the partial sums
+// meet in xor shuffles. The lanes a row gets depend only on its matrix and k,
+// never on the batch, so a request's result is bitwise the same alone and in
+// any mix.
 //
 // Each in-flight slot owns a stream, pinned host table, device table, counter
 // and a captured graph: copy the table in, reset the counter with a one-thread
@@ -31,25 +32,24 @@ namespace lsp {
 namespace {
 
 constexpr int kMaxBatch = 256;
-constexpr int kChunkNnz = 2048;
-constexpr int kChunkRows = 512;
+constexpr int kItemIters = 32;  // target loads per lane in one work item
+constexpr int kMaxPasses = 8;   // passes of 64 / G rows in one work item
 constexpr int kServeBlock = 256;
 
 struct alignas(16) ReqDesc {
   const int* rp;
   const int* ci;
   const void* val;
-  const int* chunk_row;  // [nchunks + 1] first row of each chunk
   const void* X;
   void* Y;
   int64_t ldx, ldy;
   double alpha, beta;
-  int32_t k, lanes;
-  int32_t item_begin, nchunks;
+  int32_t m, k, lanes, rows_per_item;
+  int32_t item_begin, nitems;
 };
 
 struct alignas(16) BatchHeader {
-  int32_t nreq, total_items, pad0, pad1;
+  int32_t nreq, total_items, grab, pad;  // grab: work items a wave takes per atomic
 };
 
 constexpr size_t kTableBytes = sizeof(BatchHeader) + sizeof(ReqDesc) * kMaxBatch;
@@ -73,10 +73,20 @@ __device__ __forceinline__ void run_rows(const ReqDesc& q, int r0, int r1, int l
   for (int base = r0; base < r1; base += P) {
     const int row = base + grp;
     const bool ok = row < r1 && j < k;
-    V acc = 0;
+    V acc = 0, acc2 = 0, acc3 = 0, acc4 = 0;
     if (ok) {
       const int e = q.rp[row + 1];
-      for (int p = q.rp[row] + s; p < e; p += nzl) acc += val[p] * X[(int64_t)q.ci[p] * q.ldx + j];
+      int p = q.rp[row] + s;
+      for (; p + 3 * nzl < e; p += 4 * nzl) {  // four independent chains in flight
+        const int c0 = q.ci[p], c1 = q.ci[p + nzl], c2 = q.ci[p + 2 * nzl], c3 = q.ci[p + 3 * nzl];
+        const V v0 = val[p], v1 = val[p + nzl], v2 = val[p + 2 * nzl], v3 = val[p + 3 * nzl];
+        acc += v0 * X[(int64_t)c0 * q.ldx + j];
+        acc2 += v1 * X[(int64_t)c1 * q.ldx + j];
+        acc3 += v2 * X[(int64_t)c2 * q.ldx + j];
+        acc4 += v3 * X[(int64_t)c3 * q.ldx + j];
+      }
+      for (; p < e; p += nzl) acc += val[p] * X[(int64_t)q.ci[p] * q.ldx + j];
+      acc += acc2 + (acc3 + acc4);
     }
     for (int off = kk; off < G; off <<= 1) acc += __shfl_xor(acc, off);
     if (ok && s == 0) {
@@ -92,25 +102,29 @@ __global__ __launch_bounds__(kServeBlock) void packed_kernel(const unsigned char
   __shared__ int s_begin[kMaxBatch + 1];
   const BatchHeader* H = reinterpret_cast<const BatchHeader*>(table);
   const ReqDesc* R = reinterpret_cast<const ReqDesc*>(table + sizeof(BatchHeader));
-  const int nreq = H->nreq, total = H->total_items;
+  const int nreq = H->nreq, total = H->total_items, grab = H->grab;
   for (int i = threadIdx.x; i < nreq; i += kServeBlock) s_begin[i] = R[i].item_begin;
   if (threadIdx.x == 0) s_begin[nreq] = total;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   for (;;) {
-    int item = 0;
-    if (lane == 0) item = atomicAdd(counter, 1);
-    item = __shfl(item, 0);
-    if (item >= total) break;  // every wave ends here once the items are taken
-    int lo = 0, hi = nreq - 1;  // last request whose first item <= item
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_begin[mid] <= item) lo = mid;
-      else hi = mid - 1;
+    int first = 0;
+    if (lane == 0) first = atomicAdd(counter, grab);
+    first = __shfl(first, 0);
+    if (first >= total) break;  // every wave ends here once the items are taken
+    const int last = first + grab < total ? first + grab : total;
+    for (int item = first; item < last; item++) {
+      int lo = 0, hi = nreq - 1;  // last request whose first item <= item
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_begin[mid] <= item) lo = mid;
+        else hi = mid - 1;
+      }
+      const ReqDesc& q = R[lo];
+      const int r0 = (item - s_begin[lo]) * q.rows_per_item;
+      const int r1 = r0 + q.rows_per_item < q.m ? r0 + q.rows_per_item : q.m;
+      run_rows<V>(q, r0, r1, lane);
     }
-    const ReqDesc& q = R[lo];
-    const int c = item - s_begin[lo];
-    run_rows<V>(q, q.chunk_row[c], q.chunk_row[c + 1], lane);
   }
 }
 
@@ -123,8 +137,8 @@ double now_us() {
 struct ServerMat {
   std::unique_ptr<Matrix> A;
   lespmv_matrix_t h = nullptr;  // the same object, for the C API
-  DBuf chunks;
-  int32_t nchunks = 0, lanes = 1;
+  double avg_row = 0.0;
+  int32_t lanes = 1;
   int64_t bytes = 0, last_use = 0, pending = 0;
   bool live = false;
 };
@@ -212,7 +226,6 @@ struct Server {
     used -= sm.bytes;
     sm.A.reset();
     sm.h = nullptr;
-    sm.chunks.reset();
     sm.live = false;
   }
 
@@ -227,18 +240,33 @@ struct Server {
       const Pending& p = sl.reqs[i];
       ServerMat& sm = mats[p.mat];
       Matrix& A = *sm.A;
-      R[i] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, sm.chunks.as<int>(), p.X, p.Y, p.ldx, p.ldy,
-                     p.alpha, p.beta, p.k, sm.lanes, item, sm.nchunks};
-      item += sm.nchunks;
+      const int rpi = rows_per_item(sm, p.k);
+      const int32_t nitems = (int32_t)ceil_div(A.m, rpi);
+      R[i] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, p.X, p.Y, p.ldx, p.ldy, p.alpha, p.beta,
+                     (int32_t)A.m, p.k, sm.lanes, rpi, item, nitems};
+      item += nitems;
       sm.last_use = ++tick;
     }
     H->nreq = n;
     H->total_items = item;
+    // about eight grabs per wave: small items for balance, few atomics
+    H->grab = (int32_t)std::clamp<int64_t>(item / ((int64_t)grid * (kServeBlock / 64) * 8), 1, 64);
     LSP_HIP(hipGraphLaunch(sl.exec, sl.s));
     LSP_HIP(hipEventRecord(sl.done, sl.s));
     sl.busy = true;
     batches++;
     items += item;
+  }
+
+  // rows in one work item: passes of 64 / G rows, enough passes for about
+  // kItemIters loads per lane at the matrix's mean row length
+  static int rows_per_item(const ServerMat& sm, int k) {
+    int kk = 1;
+    while (kk < k) kk <<= 1;
+    const int G = std::max(sm.lanes, kk), nzl = G / kk;
+    const double iters = std::max(1.0, std::ceil(sm.avg_row / nzl));
+    const int passes = (int)std::clamp(std::ceil(kItemIters / iters), 1.0, (double)kMaxPasses);
+    return passes * (64 / G);
   }
 
   // retire finished slots; launch queued requests on free slots
@@ -268,19 +296,6 @@ struct Server {
   }
 };
 
-std::vector<int32_t> row_chunks(const std::vector<int32_t>& rp) {
-  std::vector<int32_t> starts{0};
-  const int64_t m = (int64_t)rp.size() - 1;
-  int64_t r = 0;
-  while (r < m) {
-    int64_t e = r + 1;
-    while (e < m && e - r < kChunkRows && rp[e + 1] - rp[r] <= kChunkNnz) e++;
-    starts.push_back((int32_t)e);
-    r = e;
-  }
-  return starts;
-}
-
 }  // namespace lsp
 
 using namespace lsp;
@@ -306,7 +321,7 @@ extern "C" int lespmv_server_create(lespmv_server_t* out, int32_t dtype, int64_t
     LSP_HIP(hipGetDevice(&dev));
     hipDeviceProp_t p;
     LSP_HIP(hipGetDeviceProperties(&p, dev));
-    S->grid = p.multiProcessorCount * 2;  // 8 waves per CU, persistent
+    S->grid = p.multiProcessorCount * 8;  // 32 waves per CU, persistent
     S->slots.resize(nslots);
     for (auto& sl : S->slots) S->build_slot(sl);
     *out = reinterpret_cast<lespmv_server_t>(S.release());
@@ -324,10 +339,8 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
     LSP_REQUIRE(id && rp && (nnz == 0 || (ci && val)) && m > 0 && n > 0, LESPMV_ERR_ARG, "bad matrix arguments");
     LSP_REQUIRE(lanes == 0 || (lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0), LESPMV_ERR_ARG,
                 "lanes must be 0 (tune) or a power of two <= 64");
-    const std::vector<int32_t> hrp(rp, rp + m + 1);
-    const std::vector<int32_t> starts = row_chunks(hrp);
     const int64_t vs = vsize(S.dtype);
-    const int64_t bytes = (m + 1) * 4 + nnz * (4 + vs) + (int64_t)starts.size() * 4;
+    const int64_t bytes = (m + 1) * 4 + nnz * (4 + vs);
     LSP_REQUIRE(bytes <= S.budget, LESPMV_ERR_OOM, "matrix larger than the server's HBM budget");
     LSP_REQUIRE(S.make_room(bytes), LESPMV_ERR_OOM, "no room: every resident matrix has requests in flight");
     lespmv_matrix_t mh = nullptr;
@@ -336,9 +349,7 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
     ServerMat sm;
     sm.A.reset(reinterpret_cast<Matrix*>(mh));
     sm.h = mh;
-    sm.chunks.alloc(sizeof(int32_t) * starts.size());
-    LSP_HIP(hipMemcpy(sm.chunks.p, starts.data(), sizeof(int32_t) * starts.size(), hipMemcpyHostToDevice));
-    sm.nchunks = (int32_t)starts.size() - 1;
+    sm.avg_row = (double)nnz / (double)m;
     if (lanes == 0) {  // the autotuner's lanes per row (rows per wave = 64 / lanes) for this matrix
       int32_t count = 0;
       lespmv_tune_candidates(1 << LESPMV_CSR, S.dtype, nullptr, 0, &count);

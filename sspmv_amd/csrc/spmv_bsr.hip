@@ -145,6 +145,50 @@ __global__ __launch_bounds__(kBlock) void bsr_spmm_kernel(int64_t m, int64_t n, 
   *yp = beta == V(0) ? alpha * sum : alpha * sum + beta * *yp;
 }
 
+// BSR 4x4 FP64 SpMM on the matrix cores: one wave per block row, one
+// v_mfma_f64_4x4x4f64 per block and group of 16 right-hand sides. The
+// instruction runs four independent 4x4x4 products; product q uses the lanes
+// 4q + x + 16y (x, y in 0..3) with A[i][k] at (x, y) = (i, k), B[k][j] at
+// (j, k) and D[i][j] at (j, i) (measured on gfx950). The four products share
+// the block (A) and take four consecutive groups of 4 right-hand sides
+// (B = the X rows of the block's 4 columns).
+__global__ __launch_bounds__(kBlock) void bsr44_spmm_mfma(int64_t m, int64_t n, int64_t mb, int k,
+                                                          const int* __restrict__ rptr, const int* __restrict__ bcol,
+                                                          const double* __restrict__ data,
+                                                          const double* __restrict__ X, int64_t ldx,
+                                                          double* __restrict__ Y, int64_t ldy, double alpha,
+                                                          double beta) {
+  const int64_t brow = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  if (brow >= mb) return;  // whole waves leave together: the MFMAs below need every lane
+  const int l = threadIdx.x & 63;
+  const int x = l & 3, q = (l >> 2) & 3, y = l >> 4;
+  const int j0 = 4 * q + x, j1 = j0 + 16;
+  double acc0 = 0.0, acc1 = 0.0;
+  const int e = rptr[brow + 1];
+  for (int b = rptr[brow]; b < e; b++) {
+    const double a = data[(int64_t)b * 16 + x * 4 + y];
+    const int64_t col = (int64_t)bcol[b] * 4 + y;
+    const bool cin = col < n;
+    const double b0 = (cin && j0 < k) ? X[col * ldx + j0] : 0.0;
+    acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b0, acc0, 0, 0, 0);
+    if (k > 16) {
+      const double b1 = (cin && j1 < k) ? X[col * ldx + j1] : 0.0;
+      acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b1, acc1, 0, 0, 0);
+    }
+  }
+  const int64_t row = brow * 4 + y;
+  if (row < m) {
+    if (j0 < k) {
+      double* p = Y + row * ldy + j0;
+      *p = beta == 0.0 ? alpha * acc0 : alpha * acc0 + beta * *p;
+    }
+    if (j1 < k) {
+      double* p = Y + row * ldy + j1;
+      *p = beta == 0.0 ? alpha * acc1 : alpha * acc1 + beta * *p;
+    }
+  }
+}
+
 int bits_for(uint64_t v) {
   int b = 0;
   while (b < 64 && (v >> b)) b++;
@@ -287,9 +331,19 @@ void bsr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, 
 
 void bsr_spmm(Matrix& A, int variant, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
               int64_t ldy, hipStream_t s) {
-  (void)variant;
   auto& d = *static_cast<BsrData*>(A.fmt[LESPMV_BSR].get());
   if (A.m == 0 || k == 0) return;
+  // 4x4 blocks in FP64 with more than four right-hand sides: the block product
+  // is a dense 4x4x4 contraction, run on the matrix cores (with four or fewer,
+  // three of the instruction's four products would idle; variant VECTOR keeps
+  // the scalar kernel)
+  if (A.dtype == LESPMV_F64 && d.br == 4 && d.bc == 4 && k > 4 && variant != LESPMV_VARIANT_VECTOR) {
+    hipLaunchKernelGGL(bsr44_spmm_mfma, grid1d(d.mb, kBlock / 64), dim3(kBlock), 0, s, A.m, A.n, d.mb, k,
+                       d.row_ptr.as<int>(), d.bcol.as<int>(), d.data.as<double>(), static_cast<const double*>(X), ldx,
+                       static_cast<double*>(Y), ldy, alpha, beta);
+    LSP_LAUNCHED();
+    return;
+  }
   if (A.dtype == LESPMV_F64) spmm_dispatch<double>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
   else spmm_dispatch<float>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
 }

@@ -20,6 +20,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMergeItems = 2048;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, workgroups dealt round-robin
 
 template <int L, class V>
 __device__ __forceinline__ V group_sum(V v) {
@@ -92,19 +93,27 @@ __device__ V block_sum(V v, V* red) {
 constexpr int kItems = kMergeItems / kBlock;  // nonzeros staged per thread
 constexpr int kLongRow = 48;                     // rows longer than this in a tile are reduced by a wavefront
 
-template <class V>
-__global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int* __restrict__ rp,
-                                                           const int* __restrict__ ci, const V* __restrict__ val,
-                                                           const V* __restrict__ x, V* __restrict__ y, V alpha,
-                                                           V beta, const int* __restrict__ coord,
-                                                           int* __restrict__ carry_row, V* __restrict__ carry_val,
-                                                           V* __restrict__ head_val) {
+// Loads of data that is read once: with NT the line is marked for early
+// eviction so it does not push reused data (x) out of the L2.
+template <bool NT, class T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+// One merge-path tile b of a CSR matrix (rp, ci, val are the matrix's own
+// arrays, coord its tile starts).
+template <class V, bool NT>
+__device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restrict__ rp,
+                                           const int* __restrict__ ci, const V* __restrict__ val,
+                                           const V* __restrict__ x, V* __restrict__ y, V alpha, V beta,
+                                           const int* __restrict__ coord, int* __restrict__ carry_row,
+                                           V* __restrict__ carry_val, V* __restrict__ head_val) {
   __shared__ V prod[kMergeItems];
   __shared__ int rows[kMergeItems + 2];
   __shared__ V red[kBlock / 64];
   __shared__ int long_rows[kMergeItems / kLongRow + 2];
   __shared__ int nlong;
-  const int b = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = coord[2 * b], z0 = coord[2 * b + 1];
   const int r1 = coord[2 * b + 2], z1 = coord[2 * b + 3];
@@ -118,10 +127,10 @@ __global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int*
   for (int q = 0; q < kItems; q++) {
     const int i = tid + q * kBlock;
     const bool ok = i < nz;
-    cidx[q] = ok ? ci[z0 + i] : 0;
-    v[q] = ok ? val[z0 + i] : V(0);
+    cidx[q] = ok ? stream_load<NT>(ci + z0 + i) : 0;
+    v[q] = ok ? stream_load<NT>(val + z0 + i) : V(0);
   }
-  for (int i = tid; i <= last - r0; i += kBlock) rows[i] = rp[r0 + i];
+  for (int i = tid; i <= last - r0; i += kBlock) rows[i] = stream_load<NT>(rp + r0 + i);
   V xv[kItems];
 #pragma unroll
   for (int q = 0; q < kItems; q++) xv[q] = (tid + q * kBlock < nz) ? x[cidx[q]] : V(0);
@@ -173,6 +182,107 @@ __global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int*
     }
   } else if (tid == 0) {
     carry_row[b] = -1;
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int* __restrict__ rp,
+                                                           const int* __restrict__ ci, const V* __restrict__ val,
+                                                           const V* __restrict__ x, V* __restrict__ y, V alpha,
+                                                           V beta, const int* __restrict__ coord,
+                                                           int* __restrict__ carry_row, V* __restrict__ carry_val,
+                                                           V* __restrict__ head_val) {
+  merge_tile<V, false>(blockIdx.x, m, rp, ci, val, x, y, alpha, beta, coord, carry_row, carry_val, head_val);
+}
+
+// Column panels: the matrix split into P column ranges (P a multiple of the 8
+// XCDs), each a CSR of its own (panel-local row pointers, entries in row
+// order). Workgroups are dealt to the XCDs round-robin, so workgroup b runs on
+// XCD b % 8; it takes panel b % 8 + 8 * ((b / 8) / maxT), tile (b / 8) % maxT.
+// Each XCD thus gathers only from its own panels' slice of x, one panel after
+// the other (the grid is dispatched roughly in order), and the slice stays in
+// that XCD's L2. Each panel writes a partial y; panel_reduce adds them in
+// panel order.
+struct PanelArgs {
+  int P;
+  int64_t maxT;
+  const int64_t* nzoff;  // [P + 1] first entry of each panel
+  const int* prp;        // [P][m + 1] panel-local row pointers
+  const int* tiles;      // [P] merge tiles per panel
+  const int* coord;      // [P][2 (maxT + 1)]
+  int* carry_row;        // [P][maxT]
+};
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void panel_merge_kernel(int64_t m, PanelArgs a, const int* __restrict__ ci,
+                                                             const V* __restrict__ val, const V* __restrict__ x,
+                                                             V* __restrict__ partial, V* __restrict__ carry_val,
+                                                             V* __restrict__ head_val) {
+  const int64_t u = blockIdx.x / kXcds;
+  const int p = (int)(blockIdx.x % kXcds + kXcds * (u / a.maxT));
+  const int64_t t = u % a.maxT;
+  if (p >= a.P || t >= a.tiles[p]) return;
+  const int64_t off = a.nzoff[p];
+  merge_tile<V, true>((int)t, m, a.prp + (int64_t)p * (m + 1), ci + off, val + off, x, partial + (int64_t)p * m, V(1),
+                      V(0), a.coord + (int64_t)p * 2 * (a.maxT + 1), a.carry_row + (int64_t)p * a.maxT,
+                      carry_val + (int64_t)p * a.maxT, head_val + (int64_t)p * a.maxT);
+}
+
+template <class V>
+__global__ void panel_fixup_kernel(int64_t m, PanelArgs a, const V* __restrict__ carry_val,
+                                   const V* __restrict__ head_val, V* __restrict__ partial) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)a.P * a.maxT) return;
+  const int p = (int)(g / a.maxT);
+  const int64_t b = g % a.maxT;
+  if (b >= a.tiles[p]) return;
+  const int* coord = a.coord + (int64_t)p * 2 * (a.maxT + 1);
+  const int* rp = a.prp + (int64_t)p * (m + 1);
+  const int* crow = a.carry_row + (int64_t)p * a.maxT;
+  const V* cval = carry_val + (int64_t)p * a.maxT;
+  const int r0 = coord[2 * b], z0 = coord[2 * b + 1], r1 = coord[2 * b + 2];
+  if (!(r0 < r1 && rp[r0] < z0)) return;
+  V total = head_val[(int64_t)p * a.maxT + b];
+  for (int64_t t = b - 1; t >= 0 && crow[t] == r0; --t) total += cval[t];
+  partial[(int64_t)p * m + r0] = total;
+}
+
+template <class V>
+__global__ void panel_reduce(int64_t m, int P, const V* __restrict__ partial, V* __restrict__ y, V alpha, V beta) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  V s = 0;
+  for (int p = 0; p < P; p++) s += __builtin_nontemporal_load(partial + (int64_t)p * m + r);
+  y[r] = axpby(alpha, s, beta, y + r);
+}
+
+// per (panel, row) entry counts, each row's counts by its own thread
+__global__ void panel_count(int64_t m, int P, int64_t width, const int* __restrict__ rp, const int* __restrict__ ci,
+                            int64_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  for (int j = rp[r]; j < rp[r + 1]; j++) cnt[(int64_t)(ci[j] / width) * (m + 1) + r]++;
+}
+
+__global__ void panel_rp(int64_t n, const int64_t* __restrict__ in, int* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int)in[i];
+}
+
+template <class V>
+__global__ void panel_scatter(int64_t m, int P, int64_t width, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const V* __restrict__ val, const int64_t* __restrict__ nzoff,
+                              const int* __restrict__ prp, int* __restrict__ pci, V* __restrict__ pval) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  int fill[16];
+  for (int p = 0; p < P; p++) fill[p] = 0;
+  for (int j = rp[r]; j < rp[r + 1]; j++) {
+    const int c = ci[j];
+    const int p = (int)(c / width);
+    const int64_t d = nzoff[p] + prp[(int64_t)p * (m + 1) + r] + fill[p]++;
+    pci[d] = c;
+    pval[d] = val[j];
   }
 }
 
@@ -253,6 +363,74 @@ void merge_run(Matrix& A, double alpha, const void* x, double beta, void* y, hip
   LSP_LAUNCHED();
 }
 
+// Splits the matrix into P column panels (row order kept inside each panel).
+template <class V>
+void build_panels(Matrix& A, int P, hipStream_t s) {
+  CsrMeta& mt = A.meta;
+  const int64_t m = A.m, width = ceil_div(A.n, P);
+  DBuf cnt;
+  cnt.alloc(sizeof(int64_t) * P * (m + 1));
+  LSP_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes, s));
+  hipLaunchKernelGGL(panel_count, grid1d(m, kBlock), dim3(kBlock), 0, s, m, P, width, A.rp.as<int>(), A.ci.as<int>(),
+                     cnt.as<int64_t>());
+  LSP_LAUNCHED();
+  std::vector<int64_t> nzoff(P + 1, 0);
+  for (int p = 0; p < P; p++) nzoff[p + 1] = nzoff[p] + scan_exclusive_i64(cnt.as<int64_t>() + p * (m + 1), m + 1, s);
+  mt.p_rp.alloc(sizeof(int) * P * (m + 1));
+  hipLaunchKernelGGL(panel_rp, grid1d(P * (m + 1), kBlock), dim3(kBlock), 0, s, (int64_t)P * (m + 1),
+                     cnt.as<int64_t>(), mt.p_rp.as<int>());
+  LSP_LAUNCHED();
+  mt.p_nzoff.alloc(sizeof(int64_t) * (P + 1));
+  LSP_HIP(hipMemcpyAsync(mt.p_nzoff.p, nzoff.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
+  mt.p_ci.alloc(sizeof(int) * std::max<int64_t>(A.nnz, 1));
+  mt.p_val.alloc(sizeof(V) * std::max<int64_t>(A.nnz, 1));
+  hipLaunchKernelGGL((panel_scatter<V>), grid1d(m, kBlock), dim3(kBlock), 0, s, m, P, width, A.rp.as<int>(),
+                     A.ci.as<int>(), A.val.as<V>(), mt.p_nzoff.as<int64_t>(), mt.p_rp.as<int>(), mt.p_ci.as<int>(),
+                     mt.p_val.as<V>());
+  LSP_LAUNCHED();
+  std::vector<int> tiles(P);
+  int64_t maxT = 1;
+  for (int p = 0; p < P; p++) {
+    tiles[p] = (int)ceil_div(m + nzoff[p + 1] - nzoff[p], kMergeItems);
+    maxT = std::max<int64_t>(maxT, tiles[p]);
+  }
+  mt.p_maxT = maxT;
+  mt.p_tiles.alloc(sizeof(int) * P);
+  LSP_HIP(hipMemcpyAsync(mt.p_tiles.p, tiles.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  mt.p_coord.alloc(sizeof(int) * P * 2 * (maxT + 1));
+  for (int p = 0; p < P; p++)
+    if (tiles[p]) {
+      hipLaunchKernelGGL(merge_partition_kernel, grid1d(tiles[p] + 1, kBlock), dim3(kBlock), 0, s, (int64_t)tiles[p],
+                         m, nzoff[p + 1] - nzoff[p], mt.p_rp.as<int>() + (int64_t)p * (m + 1),
+                         mt.p_coord.as<int>() + (int64_t)p * 2 * (maxT + 1));
+      LSP_LAUNCHED();
+    }
+  mt.p_carry_row.alloc(sizeof(int) * P * maxT);
+  mt.p_carry_val.alloc(sizeof(V) * P * maxT);
+  mt.p_head.alloc(sizeof(V) * P * maxT);
+  mt.p_partial.alloc(sizeof(V) * P * m);
+  LSP_HIP(hipStreamSynchronize(s));
+  mt.panels = P;
+}
+
+template <class V>
+void panel_run(Matrix& A, int P, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  CsrMeta& mt = A.meta;
+  if (mt.panels != P) build_panels<V>(A, P, s);
+  PanelArgs a{P, mt.p_maxT, mt.p_nzoff.as<int64_t>(), mt.p_rp.as<int>(), mt.p_tiles.as<int>(), mt.p_coord.as<int>(),
+              mt.p_carry_row.as<int>()};
+  hipLaunchKernelGGL((panel_merge_kernel<V>), dim3((unsigned)(P * mt.p_maxT)), dim3(kBlock), 0, s, A.m, a,
+                     mt.p_ci.as<int>(), mt.p_val.as<V>(), static_cast<const V*>(x), mt.p_partial.as<V>(),
+                     mt.p_carry_val.as<V>(), mt.p_head.as<V>());
+  LSP_LAUNCHED();
+  hipLaunchKernelGGL((panel_fixup_kernel<V>), grid1d(P * mt.p_maxT, kBlock), dim3(kBlock), 0, s, A.m, a,
+                     mt.p_carry_val.as<V>(), mt.p_head.as<V>(), mt.p_partial.as<V>());
+  LSP_LAUNCHED();
+  hipLaunchKernelGGL((panel_reduce<V>), grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, P, mt.p_partial.as<V>(),
+                     static_cast<V*>(y), (V)alpha, (V)beta);
+  LSP_LAUNCHED();
+}
+
 template <class V, int KP>
 void launch_spmm(Matrix& A, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
                  hipStream_t s) {
@@ -316,6 +494,12 @@ void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, 
   if (variant == LESPMV_VARIANT_MERGE) {
     if (A.dtype == LESPMV_F64) merge_run<double>(A, alpha, x, beta, y, s);
     else merge_run<float>(A, alpha, x, beta, y, s);
+    return;
+  }
+  if (variant == LESPMV_VARIANT_PANEL || variant == LESPMV_VARIANT_PANEL16) {
+    const int P = variant == LESPMV_VARIANT_PANEL ? 8 : 16;
+    if (A.dtype == LESPMV_F64) panel_run<double>(A, P, alpha, x, beta, y, s);
+    else panel_run<float>(A, P, alpha, x, beta, y, s);
     return;
   }
   // vector kernel; variants >= 16 request an explicit lane count (16 + log2 L)

@@ -118,12 +118,19 @@ def save_db(path, rows):
 
 def load_db(path):
     with np.load(path, allow_pickle=False) as z:
-        return {k: z[k] for k in z.files}
+        db = {k: z[k] for k in z.files}
+    # inputs written before non-finite values were mapped to 0 in
+    # lespmv_selector_inputs (empty matrices): the same mapping
+    db["X"] = np.where(np.isfinite(db["X"]), db["X"], 0.0).astype(np.float32)
+    return db
 
 
-def labels(times: np.ndarray) -> np.ndarray:
+def labels(times: np.ndarray, tie: float = 0.0) -> np.ndarray:
     """(n, 6) labels: class of the fastest grid candidate, then per parameter
-    head the fastest candidate of that class (-1 when none of it applies)."""
+    head the fastest candidate of that class (-1 when none of it applies).
+    With tie > 0, candidates within a factor (1 + tie) of the fastest count as
+    equally fast and the label goes to the one whose class wins most often
+    overall, so near-equal timings do not scatter the labels."""
     cf, hc, hs = lib.selector_layout()
     g = times[:, :N_GRID]
     cls_of = np.zeros(N_GRID, dtype=np.int32)
@@ -131,6 +138,11 @@ def labels(times: np.ndarray) -> np.ndarray:
         cls_of[cf[c]:cf[c + 1]] = c
     L = np.full((len(g), 6), -1, dtype=np.int32)
     best = np.argmin(g, axis=1)
+    if tie > 0:
+        prior = np.bincount(cls_of[best], minlength=8).astype(np.float64)
+        near = g <= (1.0 + tie) * g[np.arange(len(g)), best][:, None]
+        score = np.where(near, prior[cls_of][None, :] - g / g.max(), -np.inf)
+        best = np.argmax(score, axis=1)
     L[:, 0] = cls_of[best]
     for c in range(8):
         h = hc[c]
@@ -208,16 +220,16 @@ def split(n, frac=0.8, seed=3):
     return idx[:k], idx[k:]
 
 
-def run_training(db_path, model_out, report_out=None, epochs=400, seed=7, log=print):
+def run_training(db_path, model_out, report_out=None, epochs=400, seed=7, tie=0.0, log=print):
     db = load_db(db_path)
     X, times = db["X"].astype(np.float32), db["times"]
-    L = labels(times)
+    L = labels(times, tie)
     tr, te = split(len(X))
     mlp, hist = train(X[tr], L[tr], epochs=epochs, seed=seed)
     mlp.save(model_out)
     cand = picks(mlp, X[te], db["allowed"][te])
     res = {
-        "matrices": int(len(X)), "train": int(len(tr)), "test": int(len(te)), "epochs": epochs,
+        "matrices": int(len(X)), "train": int(len(tr)), "test": int(len(te)), "epochs": epochs, "tie": tie,
         "final_loss": float(hist[-1]), "first_loss": float(hist[0]),
         "selector": evaluate(times[te], cand),
         "default_csr": evaluate(times[te], np.full(len(te), N_GRID)),
@@ -258,19 +270,21 @@ def main(argv=None):
     t.add_argument("--db", required=True, nargs="+")
     t.add_argument("--model", required=True)
     t.add_argument("--report")
+    t.add_argument("--db-out", help="where to write the merged database")
     t.add_argument("--epochs", type=int, default=400)
+    t.add_argument("--tie", type=float, default=0.0)
     a = ap.parse_args(argv)
     lib.check(lib.load().lespmv_init(0))
     if a.cmd == "collect":
         collect(a.count, a.out, a.seed, a.iters, a.max_nnz, (1e4, a.n_max))
     else:
         path = a.db[0]
-        if len(a.db) > 1:  # merge several databases
+        if len(a.db) > 1 or a.db_out:  # merge several databases
             parts = [load_db(p) for p in a.db]
             merged = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
-            path = a.model + ".db.npz"
+            path = a.db_out or a.model + ".db.npz"
             np.savez_compressed(path, **merged)
-        run_training(path, a.model, a.report, a.epochs)
+        run_training(path, a.model, a.report, a.epochs, tie=a.tie)
 
 
 if __name__ == "__main__":
