@@ -95,7 +95,7 @@ int main(int argc, char** argv) {
   }
   CHECK(lespmv_init(0));
   lespmv_server_t S;
-  CHECK(lespmv_server_create(&S, LESPMV_F64, (int64_t)(budget_gb * 1e9), 128, 3));
+  CHECK(lespmv_server_create(&S, LESPMV_F64, (int64_t)(budget_gb * 1e9), 256, 4));
 
   // ---- resident matrices and per-call handles ----
   std::vector<Mat> mats(nmat);

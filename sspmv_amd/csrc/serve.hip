@@ -154,11 +154,13 @@ struct Pending {
   double t_submit;
 };
 
+constexpr int kGraphs = 3;  // persistent grids of three sizes per slot
+
 struct Slot {
   hipStream_t s = nullptr;
   hipEvent_t done = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[kGraphs] = {};
+  hipGraphExec_t exec[kGraphs] = {};
   unsigned char* h_table = nullptr;
   DBuf d_table, counter;
   std::vector<Pending> reqs;
@@ -169,7 +171,8 @@ struct Server {
   int dtype = LESPMV_F64;
   int64_t budget = 0, used = 0;
   int32_t max_batch = kMaxBatch;
-  int grid = 0;
+  int grid = 0;              // the largest persistent grid (blocks)
+  int grids[kGraphs] = {};   // grid sizes of a slot's graphs, smallest first
   std::vector<ServerMat> mats;
   std::deque<Pending> queue;
   std::vector<Slot> slots;
@@ -180,8 +183,10 @@ struct Server {
   ~Server() {
     for (auto& sl : slots) {
       if (sl.s) (void)hipStreamSynchronize(sl.s);
-      if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
-      if (sl.graph) (void)hipGraphDestroy(sl.graph);
+      for (int g = 0; g < kGraphs; g++) {
+        if (sl.exec[g]) (void)hipGraphExecDestroy(sl.exec[g]);
+        if (sl.graph[g]) (void)hipGraphDestroy(sl.graph[g]);
+      }
       if (sl.done) (void)hipEventDestroy(sl.done);
       if (sl.h_table) (void)hipHostFree(sl.h_table);
       if (sl.s) (void)hipStreamDestroy(sl.s);
@@ -195,17 +200,19 @@ struct Server {
     std::memset(sl.h_table, 0, kTableBytes);
     sl.d_table.alloc(kTableBytes);
     sl.counter.alloc(sizeof(int));
-    LSP_HIP(hipStreamBeginCapture(sl.s, hipStreamCaptureModeThreadLocal));
-    LSP_HIP(hipMemcpyAsync(sl.d_table.p, sl.h_table, kTableBytes, hipMemcpyHostToDevice, sl.s));
-    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, sl.s, sl.counter.as<int>());
-    if (dtype == LESPMV_F64)
-      hipLaunchKernelGGL(packed_kernel<double>, dim3(grid), dim3(kServeBlock), 0, sl.s,
-                         sl.d_table.as<unsigned char>(), sl.counter.as<int>());
-    else
-      hipLaunchKernelGGL(packed_kernel<float>, dim3(grid), dim3(kServeBlock), 0, sl.s,
-                         sl.d_table.as<unsigned char>(), sl.counter.as<int>());
-    LSP_HIP(hipStreamEndCapture(sl.s, &sl.graph));
-    LSP_HIP(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
+    for (int g = 0; g < kGraphs; g++) {
+      LSP_HIP(hipStreamBeginCapture(sl.s, hipStreamCaptureModeThreadLocal));
+      LSP_HIP(hipMemcpyAsync(sl.d_table.p, sl.h_table, kTableBytes, hipMemcpyHostToDevice, sl.s));
+      hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, sl.s, sl.counter.as<int>());
+      if (dtype == LESPMV_F64)
+        hipLaunchKernelGGL(packed_kernel<double>, dim3(grids[g]), dim3(kServeBlock), 0, sl.s,
+                           sl.d_table.as<unsigned char>(), sl.counter.as<int>());
+      else
+        hipLaunchKernelGGL(packed_kernel<float>, dim3(grids[g]), dim3(kServeBlock), 0, sl.s,
+                           sl.d_table.as<unsigned char>(), sl.counter.as<int>());
+      LSP_HIP(hipStreamEndCapture(sl.s, &sl.graph[g]));
+      LSP_HIP(hipGraphInstantiate(&sl.exec[g], sl.graph[g], nullptr, nullptr, 0));
+    }
   }
 
   // evict idle matrices, least recently used first, until `need` bytes fit
@@ -249,9 +256,12 @@ struct Server {
     }
     H->nreq = n;
     H->total_items = item;
-    // about eight grabs per wave: small items for balance, few atomics
-    H->grab = (int32_t)std::clamp<int64_t>(item / ((int64_t)grid * (kServeBlock / 64) * 8), 1, 64);
-    LSP_HIP(hipGraphLaunch(sl.exec, sl.s));
+    // the smallest grid that still gives every wave several items; about
+    // eight grabs per wave: small items for balance, few atomics
+    int g = 0;
+    while (g + 1 < kGraphs && (int64_t)grids[g] * (kServeBlock / 64) * 4 < item) g++;
+    H->grab = (int32_t)std::clamp<int64_t>(item / ((int64_t)grids[g] * (kServeBlock / 64) * 8), 1, 64);
+    LSP_HIP(hipGraphLaunch(sl.exec[g], sl.s));
     LSP_HIP(hipEventRecord(sl.done, sl.s));
     sl.busy = true;
     batches++;
@@ -322,6 +332,9 @@ extern "C" int lespmv_server_create(lespmv_server_t* out, int32_t dtype, int64_t
     hipDeviceProp_t p;
     LSP_HIP(hipGetDeviceProperties(&p, dev));
     S->grid = p.multiProcessorCount * 8;  // 32 waves per CU, persistent
+    S->grids[0] = std::max(1, p.multiProcessorCount / 4);
+    S->grids[1] = p.multiProcessorCount * 2;
+    S->grids[2] = S->grid;
     S->slots.resize(nslots);
     for (auto& sl : S->slots) S->build_slot(sl);
     *out = reinterpret_cast<lespmv_server_t>(S.release());

@@ -154,6 +154,47 @@ def labels(times: np.ndarray, tie: float = 0.0) -> np.ndarray:
     return L
 
 
+# The reference's nine labels (LeSpMV/script/write_label.py:63-75).
+REFERENCE_LABELS = ["COO", "CSR", "DIA", "ELL", "S-ELL", "S-ELL-sigma", "S-ELL-R", "CSR5", "BSR"]
+
+
+def reference_format_times(times: np.ndarray) -> np.ndarray:
+    """(n, 9) best time per reference format label (inf where none applies):
+    every grid candidate belongs to one label (both CSR kernels to CSR)."""
+    grid = lib.tune_candidates()
+    fmt_label = {lib.COO: 0, lib.CSR: 1, lib.DIA: 2, lib.ELL: 3, lib.SELL: 4, lib.SELL_C_SIGMA: 5,
+                 lib.SELL_C_R: 6, lib.CSR5: 7, lib.BSR: 8}
+    out = np.full((len(times), 9), np.inf)
+    for j in range(N_GRID):
+        lab = fmt_label[grid[j].fmt]
+        out[:, lab] = np.minimum(out[:, lab], times[:, j])
+    return out
+
+
+def probability_vectors(times: np.ndarray, nnz: np.ndarray) -> np.ndarray:
+    """The reference's probability label: each format's best GFLOP/s divided
+    by the sum over the nine formats (0 for formats that do not apply)."""
+    t = reference_format_times(times)
+    gflops = np.where(np.isfinite(t), 2.0 * np.asarray(nnz, dtype=np.float64)[:, None] / (t * 1e6), 0.0)
+    s = gflops.sum(axis=1, keepdims=True)
+    return gflops / np.where(s > 0, s, 1.0)
+
+
+def write_label_files(root: str, name: str, times_row: np.ndarray, nnz: int) -> str:
+    """<root>/<name>/<name>.format_label ("<FORMAT> <id>") and .prob_label
+    ("<best FORMAT> p_COO ... p_BSR"), as write_label.py lays them out
+    (LeSpMV/script/write_label.py:14-61)."""
+    p = probability_vectors(times_row[None, :], np.array([nnz]))[0]
+    best = int(np.argmax(p))
+    d = os.path.join(root, name)
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, name + ".format_label"), "w") as f:
+        f.write(f"{REFERENCE_LABELS[best]} {best}\n")
+    with open(os.path.join(d, name + ".prob_label"), "w") as f:
+        f.write(" ".join([REFERENCE_LABELS[best]] + [repr(float(v)) for v in p]) + "\n")
+    return d
+
+
 def standardiser(X):
     mean = X.mean(axis=0)
     std = X.std(axis=0)

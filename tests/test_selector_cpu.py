@@ -38,3 +38,25 @@ def test_sampling_covers_families_and_respects_nnz_cap():
         if fam == "blockdiag":
             assert params[0] * params[1] <= 6e6 * 1.0001 or params[1] == 2
     assert fams == set(selector.FAMILIES)
+
+
+def test_reference_labels_and_probability_vector(tmp_path):
+    """Grid timings folded into the reference's nine labels; the probability
+    label is each format's best GFLOP/s over their sum."""
+    cf, _, _ = lib.selector_layout()
+    t = np.full((1, 28), np.inf)
+    t[0, cf[0] + 1] = 2.0      # CSR vector
+    t[0, cf[1]] = 1.0          # CSR merge: CSR's best is 1.0
+    t[0, cf[5]] = 4.0          # DIA
+    grid = lib.tune_candidates()
+    sell = [j for j in range(27) if grid[j].fmt == lib.SELL_C_SIGMA][0]
+    t[0, sell] = 2.0
+    ft = selector.reference_format_times(t)[0]
+    assert ft[1] == 1.0 and ft[2] == 4.0 and ft[5] == 2.0 and np.isinf(ft[0])
+    p = selector.probability_vectors(t, np.array([1000]))[0]
+    g = np.array([0, 1 / 1.0, 1 / 4.0, 0, 0, 1 / 2.0, 0, 0, 0])
+    np.testing.assert_allclose(p, g / g.sum())
+    d = selector.write_label_files(str(tmp_path), "m", t[0], 1000)
+    assert open(f"{d}/m.format_label").read() == "CSR 1\n"
+    toks = open(f"{d}/m.prob_label").read().split()
+    assert toks[0] == "CSR" and len(toks) == 10
