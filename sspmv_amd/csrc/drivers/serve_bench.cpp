@@ -13,7 +13,8 @@
 //     LeSpMV_csr usage), (b) all calls issued asynchronously on the matrices'
 //     own streams and one synchronisation at the end (its throughput bound).
 // Prints one JSON object on stdout, progress on stderr.
-// Options: --matrices=N --requests=N --rates=r1,r2,... --seed=S --budget-gb=G
+// Options: --matrices=N --requests=N --rates=r1,r2,... --seed=S --budget-gb=G --slots=N
+//          --families=0123 (0 2-D Laplacian, 1 27-point stencil, 2 lognormal rows, 3 4x4 blocks)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -82,6 +83,14 @@ int main(int argc, char** argv) {
   const int nreq = std::stoi(arg(argc, argv, "--requests", "4000"));
   const uint64_t seed = std::stoull(arg(argc, argv, "--seed", "1000"));
   const double budget_gb = std::stod(arg(argc, argv, "--budget-gb", "200"));
+  const int nslots = std::stoi(arg(argc, argv, "--slots", "4"));
+  std::vector<int> families;  // generator families to draw from (default all four)
+  {
+    const std::string f = arg(argc, argv, "--families", "0,1,2,3");
+    for (char c : f)
+      if (c >= '0' && c <= '3') families.push_back(c - '0');
+    if (families.empty()) families = {0, 1, 2, 3};
+  }
   std::vector<double> rates;
   {
     const std::string r = arg(argc, argv, "--rates", "5000,20000,50000");
@@ -95,7 +104,7 @@ int main(int argc, char** argv) {
   }
   CHECK(lespmv_init(0));
   lespmv_server_t S;
-  CHECK(lespmv_server_create(&S, LESPMV_F64, (int64_t)(budget_gb * 1e9), 256, 4));
+  CHECK(lespmv_server_create(&S, LESPMV_F64, (int64_t)(budget_gb * 1e9), 256, nslots));
 
   // ---- resident matrices and per-call handles ----
   std::vector<Mat> mats(nmat);
@@ -104,7 +113,7 @@ int main(int argc, char** argv) {
   for (int i = 0; i < nmat; i++) {
     const uint64_t h = mix(seed * 7919 + (uint64_t)i);
     const double rows = std::exp(std::log(1e3) + unit(h) * (std::log(2e5) - std::log(1e3)));
-    const int fam = (int)(mix(h) % 4);
+    const int fam = families[mix(h) % families.size()];
     std::vector<double> p;
     int gen;
     if (fam == 0) {
@@ -155,8 +164,15 @@ int main(int argc, char** argv) {
     reqs[r].mat = (int)(h % (uint64_t)nmat);
     reqs[r].k = 1 + (int)(mix(h) % 32);
   }
+  // flops, and the bytes a request must move at least: its CSR matrix
+  // (row pointers, indices, values), X once and Y once
   int64_t flops = 0;
-  for (const auto& q : reqs) flops += 2 * mats[q.mat].nnz * q.k;
+  double bytes = 0;
+  for (const auto& q : reqs) {
+    const Mat& M = mats[q.mat];
+    flops += 2 * M.nnz * q.k;
+    bytes += 4.0 * (double)(M.m + 1) + 12.0 * (double)M.nnz + 8.0 * (double)q.k * (double)(M.n + M.m);
+  }
 
   auto run_server = [&](double rate) {
     // Poisson arrivals at `rate` req/s, or everything at t = 0 (saturation)
@@ -191,9 +207,9 @@ int main(int argc, char** argv) {
     char buf[512];
     std::snprintf(buf, sizeof buf,
                   "{\"offered_req_per_s\": %s, \"req_per_s\": %.1f, \"p50_us\": %.1f, \"p99_us\": %.1f, "
-                  "\"mean_us\": %.1f, \"gflops\": %.1f}",
+                  "\"mean_us\": %.1f, \"gflops\": %.1f, \"GBs\": %.1f}",
                   rate > 0 ? std::to_string((long long)rate).c_str() : "\"saturation\"", nreq / secs,
-                  pct(lat, 0.5), pct(lat, 0.99), mean(lat), flops / secs * 1e-9);
+                  pct(lat, 0.5), pct(lat, 0.99), mean(lat), flops / secs * 1e-9, bytes / secs * 1e-9);
     std::fprintf(stderr, "server %s\n", buf);
     return std::string(buf);
   };
@@ -210,8 +226,9 @@ int main(int argc, char** argv) {
     }
     const double secs = (now_us() - t0) * 1e-6;
     char buf[384];
-    std::snprintf(buf, sizeof buf, "{\"req_per_s\": %.1f, \"p50_us\": %.1f, \"p99_us\": %.1f, \"gflops\": %.1f}",
-                  nreq / secs, pct(lat, 0.5), pct(lat, 0.99), flops / secs * 1e-9);
+    std::snprintf(buf, sizeof buf,
+                  "{\"req_per_s\": %.1f, \"p50_us\": %.1f, \"p99_us\": %.1f, \"gflops\": %.1f, \"GBs\": %.1f}",
+                  nreq / secs, pct(lat, 0.5), pct(lat, 0.99), flops / secs * 1e-9, bytes / secs * 1e-9);
     std::fprintf(stderr, "per-call sync %s\n", buf);
     return std::string(buf);
   };
@@ -227,7 +244,8 @@ int main(int argc, char** argv) {
     CHECK(lespmv_synchronize());
     const double secs = (now_us() - t0) * 1e-6;
     char buf[256];
-    std::snprintf(buf, sizeof buf, "{\"req_per_s\": %.1f, \"gflops\": %.1f}", nreq / secs, flops / secs * 1e-9);
+    std::snprintf(buf, sizeof buf, "{\"req_per_s\": %.1f, \"gflops\": %.1f, \"GBs\": %.1f}", nreq / secs,
+                  flops / secs * 1e-9, bytes / secs * 1e-9);
     std::fprintf(stderr, "per-call async %s\n", buf);
     return std::string(buf);
   };
@@ -279,10 +297,10 @@ int main(int argc, char** argv) {
   const std::string async = run_percall_async();
   CHECK(lespmv_server_stats(S, &st));
 
-  std::printf("{\"matrices\": %d, \"requests\": %d, \"total_rows\": %lld, \"total_nnz\": %lld, "
+  std::printf("{\"matrices\": %d, \"requests\": %d, \"bytes_per_request\": %.0f, \"total_rows\": %lld, \"total_nnz\": %lld, "
               "\"resident_bytes\": %lld, \"build_s\": %.1f, \"batches\": %lld, \"server_saturation\": %s, "
               "\"server_poisson\": [",
-              nmat, nreq, (long long)total_rows, (long long)total_nnz, (long long)st.bytes_used, build_s,
+              nmat, nreq, bytes / nreq, (long long)total_rows, (long long)total_nnz, (long long)st.bytes_used, build_s,
               (long long)st.batches, sat.c_str());
   for (size_t i = 0; i < loads.size(); i++) std::printf("%s%s", i ? ", " : "", loads[i].c_str());
   std::printf("], \"unloaded\": %s, \"percall_sync\": %s, \"percall_async\": %s}\n", unloaded.c_str(),

@@ -35,6 +35,8 @@ constexpr int kMaxBatch = 256;
 constexpr int kItemIters = 32;  // target loads per lane in one work item
 constexpr int kMaxPasses = 8;   // passes of 64 / G rows in one work item
 constexpr int kServeBlock = 256;
+constexpr int kChains = 8;  // nonzeros per lane whose loads are issued together
+constexpr int kModeW[5] = {1, 2, 2, 4, 4};  // right-hand sides per lane of each request mode
 
 struct alignas(16) ReqDesc {
   const int* rp;
@@ -46,6 +48,7 @@ struct alignas(16) ReqDesc {
   double alpha, beta;
   int32_t m, k, lanes, rows_per_item;
   int32_t item_begin, nitems;
+  int32_t mode, pad[3];  // right-hand sides per lane and vector access, see rhs_mode
 };
 
 struct alignas(16) BatchHeader {
@@ -58,40 +61,110 @@ __global__ void reset_counter(int* c) {
   if (threadIdx.x == 0) *c = 0;
 }
 
-template <class V>
+// Pointers taken from a request descriptor are generic to the compiler; in the
+// global address space it issues global (not flat) loads, which wait on the
+// vector-memory counter alone.
+#define GLOBAL_AS __attribute__((address_space(1)))
+
+// X and Y values of one lane's W right-hand sides: consecutive in memory; one
+// vector access when the request is aligned for it (VEC), else W guarded
+// scalar accesses.
+template <class V, int W, bool VEC>
+__device__ __forceinline__ void load_w(const GLOBAL_AS V* __restrict__ p, int nvalid, V* out) {
+  if constexpr (VEC && W > 1) {
+    typedef V vec_t __attribute__((ext_vector_type(W)));
+    const vec_t v = *(const GLOBAL_AS vec_t*)p;
+#pragma unroll
+    for (int u = 0; u < W; u++) out[u] = v[u];
+  } else {
+#pragma unroll
+    for (int u = 0; u < W; u++) out[u] = u < nvalid ? p[u] : (V)0;
+  }
+}
+
+// Rows [r0, r1) of request q for all k right-hand sides. A row gets G lanes
+// (G = max(lanes, kk), kk = lanes needed to cover k at W right-hand sides
+// each, rounded up to a power of two); lane t handles right-hand sides
+// (t mod kk)*W .. +W-1 and every (G/kk)-th nonzero of the row, and the
+// partial sums meet in xor shuffles. A lane issues the index, value and x
+// loads of kChains nonzeros before it uses any of them (out-of-row slots are
+// masked), so that many gathers are in flight per lane.
+template <class V, int W, bool VEC>
 __device__ __forceinline__ void run_rows(const ReqDesc& q, int r0, int r1, int lane) {
+  constexpr int CH = kChains;
   const int k = q.k;
+  const int kw = (k + W - 1) / W;
   int kk = 1;
-  while (kk < k) kk <<= 1;
+  while (kk < kw) kk <<= 1;
   const int G = q.lanes > kk ? q.lanes : kk;  // lanes per row (power of two <= 64)
   const int P = 64 / G;                       // rows per pass
-  const int grp = lane / G, t = lane % G, j = t % kk, s = t / kk, nzl = G / kk;
-  const V* __restrict__ val = static_cast<const V*>(q.val);
-  const V* __restrict__ X = static_cast<const V*>(q.X);
-  V* __restrict__ Y = static_cast<V*>(q.Y);
+  const int grp = lane / G, t = lane % G, j = (t % kk) * W, s = t / kk, nzl = G / kk;
+  const int nvalid = k - j;
+  const GLOBAL_AS V* __restrict__ val = (const GLOBAL_AS V*)q.val;
+  const GLOBAL_AS V* __restrict__ X = (const GLOBAL_AS V*)q.X + j;
+  GLOBAL_AS V* __restrict__ Y = (GLOBAL_AS V*)q.Y;
+  const GLOBAL_AS int* __restrict__ rp = (const GLOBAL_AS int*)q.rp;
+  const GLOBAL_AS int* __restrict__ ci = (const GLOBAL_AS int*)q.ci;
+  const int64_t ldx = q.ldx;
   const V alpha = (V)q.alpha, beta = (V)q.beta;
   for (int base = r0; base < r1; base += P) {
     const int row = base + grp;
     const bool ok = row < r1 && j < k;
-    V acc = 0, acc2 = 0, acc3 = 0, acc4 = 0;
+    V acc[W];
+#pragma unroll
+    for (int u = 0; u < W; u++) acc[u] = 0;
     if (ok) {
-      const int e = q.rp[row + 1];
-      int p = q.rp[row] + s;
-      for (; p + 3 * nzl < e; p += 4 * nzl) {  // four independent chains in flight
-        const int c0 = q.ci[p], c1 = q.ci[p + nzl], c2 = q.ci[p + 2 * nzl], c3 = q.ci[p + 3 * nzl];
-        const V v0 = val[p], v1 = val[p + nzl], v2 = val[p + 2 * nzl], v3 = val[p + 3 * nzl];
-        acc += v0 * X[(int64_t)c0 * q.ldx + j];
-        acc2 += v1 * X[(int64_t)c1 * q.ldx + j];
-        acc3 += v2 * X[(int64_t)c2 * q.ldx + j];
-        acc4 += v3 * X[(int64_t)c3 * q.ldx + j];
+      const int e = rp[row + 1];
+      for (int p = rp[row] + s; p < e; p += CH * nzl) {
+        int cc[CH];
+        V vv[CH], xx[CH][W];
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+          const int pc = p + c * nzl;
+          cc[c] = pc < e ? ci[pc] : 0;
+          vv[c] = pc < e ? val[pc] : (V)0;
+        }
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+          if (p + c * nzl < e) {
+            load_w<V, W, VEC>(X + (int64_t)cc[c] * ldx, nvalid, xx[c]);
+          } else {
+#pragma unroll
+            for (int u = 0; u < W; u++) xx[c][u] = 0;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CH; c++)
+#pragma unroll
+          for (int u = 0; u < W; u++) acc[u] += vv[c] * xx[c][u];
       }
-      for (; p < e; p += nzl) acc += val[p] * X[(int64_t)q.ci[p] * q.ldx + j];
-      acc += acc2 + (acc3 + acc4);
     }
-    for (int off = kk; off < G; off <<= 1) acc += __shfl_xor(acc, off);
+    V sum[W];
+#pragma unroll
+    for (int u = 0; u < W; u++) {
+      V a = acc[u];
+      for (int off = kk; off < G; off <<= 1) a += __shfl_xor(a, off);
+      sum[u] = a;
+    }
     if (ok && s == 0) {
-      V* y = Y + (int64_t)row * q.ldy + j;
-      *y = beta == (V)0 ? alpha * acc : alpha * acc + beta * *y;
+      GLOBAL_AS V* y = Y + (int64_t)row * q.ldy + j;
+      if constexpr (VEC && W > 1) {
+        typedef V vec_t __attribute__((ext_vector_type(W)));
+        vec_t o;
+        if (beta == (V)0) {
+#pragma unroll
+          for (int u = 0; u < W; u++) o[u] = alpha * sum[u];
+        } else {
+          const vec_t old = *(const GLOBAL_AS vec_t*)y;
+#pragma unroll
+          for (int u = 0; u < W; u++) o[u] = alpha * sum[u] + beta * old[u];
+        }
+        *(GLOBAL_AS vec_t*)y = o;
+      } else {
+#pragma unroll
+        for (int u = 0; u < W; u++)
+          if (u < nvalid) y[u] = beta == (V)0 ? alpha * sum[u] : alpha * sum[u] + beta * y[u];
+      }
     }
   }
 }
@@ -110,20 +183,28 @@ __global__ __launch_bounds__(kServeBlock) void packed_kernel(const unsigned char
   for (;;) {
     int first = 0;
     if (lane == 0) first = atomicAdd(counter, grab);
-    first = __shfl(first, 0);
+    first = __builtin_amdgcn_readfirstlane(first);  // lane 0's value, wave-uniform from here on
     if (first >= total) break;  // every wave ends here once the items are taken
     const int last = first + grab < total ? first + grab : total;
+    int lo = 0, hi = nreq - 1;  // last request whose first item <= first
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_begin[mid] <= first) lo = mid;
+      else hi = mid - 1;
+    }
     for (int item = first; item < last; item++) {
-      int lo = 0, hi = nreq - 1;  // last request whose first item <= item
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_begin[mid] <= item) lo = mid;
-        else hi = mid - 1;
-      }
+      while (lo + 1 < nreq && s_begin[lo + 1] <= item) lo++;  // items of a grab are consecutive
+      lo = __builtin_amdgcn_readfirstlane(lo);
       const ReqDesc& q = R[lo];
       const int r0 = (item - s_begin[lo]) * q.rows_per_item;
       const int r1 = r0 + q.rows_per_item < q.m ? r0 + q.rows_per_item : q.m;
-      run_rows<V>(q, r0, r1, lane);
+      switch (q.mode) {
+        case 0: run_rows<V, 1, false>(q, r0, r1, lane); break;
+        case 1: run_rows<V, 2, false>(q, r0, r1, lane); break;
+        case 2: run_rows<V, 2, true>(q, r0, r1, lane); break;
+        case 3: run_rows<V, 4, false>(q, r0, r1, lane); break;
+        default: run_rows<V, 4, true>(q, r0, r1, lane); break;
+      }
     }
   }
 }
@@ -161,6 +242,7 @@ struct Slot {
   hipEvent_t done = nullptr;
   hipGraph_t graph[kGraphs] = {};
   hipGraphExec_t exec[kGraphs] = {};
+  hipGraphNode_t copy[kGraphs] = {};  // the table copy of each graph, resized per launch
   unsigned char* h_table = nullptr;
   DBuf d_table, counter;
   std::vector<Pending> reqs;
@@ -211,6 +293,16 @@ struct Server {
         hipLaunchKernelGGL(packed_kernel<float>, dim3(grids[g]), dim3(kServeBlock), 0, sl.s,
                            sl.d_table.as<unsigned char>(), sl.counter.as<int>());
       LSP_HIP(hipStreamEndCapture(sl.s, &sl.graph[g]));
+      size_t nn = 0;
+      LSP_HIP(hipGraphGetNodes(sl.graph[g], nullptr, &nn));
+      std::vector<hipGraphNode_t> nodes(nn);
+      LSP_HIP(hipGraphGetNodes(sl.graph[g], nodes.data(), &nn));
+      for (auto nd : nodes) {
+        hipGraphNodeType ty;
+        LSP_HIP(hipGraphNodeGetType(nd, &ty));
+        if (ty == hipGraphNodeTypeMemcpy) sl.copy[g] = nd;
+      }
+      LSP_REQUIRE(sl.copy[g], LESPMV_ERR_DEVICE, "captured serving graph has no table copy");
       LSP_HIP(hipGraphInstantiate(&sl.exec[g], sl.graph[g], nullptr, nullptr, 0));
     }
   }
@@ -247,20 +339,26 @@ struct Server {
       const Pending& p = sl.reqs[i];
       ServerMat& sm = mats[p.mat];
       Matrix& A = *sm.A;
-      const int rpi = rows_per_item(sm, p.k);
+      const int mode = rhs_mode(p);
+      const int rpi = rows_per_item(sm, p.k, kModeW[mode]);
       const int32_t nitems = (int32_t)ceil_div(A.m, rpi);
       R[i] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, p.X, p.Y, p.ldx, p.ldy, p.alpha, p.beta,
-                     (int32_t)A.m, p.k, sm.lanes, rpi, item, nitems};
+                     (int32_t)A.m, p.k, sm.lanes, rpi, item, nitems, mode, {0, 0, 0}};
       item += nitems;
       sm.last_use = ++tick;
     }
     H->nreq = n;
     H->total_items = item;
-    // the smallest grid that still gives every wave several items; about
-    // eight grabs per wave: small items for balance, few atomics
+    // the smallest grid with at least one wave per item (a short batch finishes
+    // in about one item's time); about eight grabs per wave: small items for
+    // balance, few atomics
     int g = 0;
-    while (g + 1 < kGraphs && (int64_t)grids[g] * (kServeBlock / 64) * 4 < item) g++;
+    while (g + 1 < kGraphs && (int64_t)grids[g] * (kServeBlock / 64) < item) g++;
     H->grab = (int32_t)std::clamp<int64_t>(item / ((int64_t)grids[g] * (kServeBlock / 64) * 8), 1, 64);
+    // copy only the descriptors in use (the slot's previous launch has finished)
+    LSP_HIP(hipGraphExecMemcpyNodeSetParams1D(sl.exec[g], sl.copy[g], sl.d_table.p, sl.h_table,
+                                              sizeof(BatchHeader) + sizeof(ReqDesc) * (size_t)n,
+                                              hipMemcpyHostToDevice));
     LSP_HIP(hipGraphLaunch(sl.exec[g], sl.s));
     LSP_HIP(hipEventRecord(sl.done, sl.s));
     sl.busy = true;
@@ -268,11 +366,24 @@ struct Server {
     items += item;
   }
 
+  // right-hand sides per lane (W) and whether a lane's W values move as one
+  // vector: that needs k, ldx and ldy multiples of W and X, Y aligned to the
+  // vector. Depends on the request alone, never on the batch.
+  int rhs_mode(const Pending& p) const {
+    const int W = p.k == 1 ? 1 : p.k <= 3 ? 2 : 4;
+    if (W == 1) return 0;
+    const uintptr_t align = (uintptr_t)W * (uintptr_t)vsize(dtype);
+    const bool vec = p.k % W == 0 && p.ldx % W == 0 && p.ldy % W == 0 &&
+                     reinterpret_cast<uintptr_t>(p.X) % align == 0 && reinterpret_cast<uintptr_t>(p.Y) % align == 0;
+    return W == 2 ? (vec ? 2 : 1) : (vec ? 4 : 3);
+  }
+
   // rows in one work item: passes of 64 / G rows, enough passes for about
   // kItemIters loads per lane at the matrix's mean row length
-  static int rows_per_item(const ServerMat& sm, int k) {
+  static int rows_per_item(const ServerMat& sm, int k, int W) {
+    const int kw = (k + W - 1) / W;
     int kk = 1;
-    while (kk < k) kk <<= 1;
+    while (kk < kw) kk <<= 1;
     const int G = std::max(sm.lanes, kk), nzl = G / kk;
     const double iters = std::max(1.0, std::ceil(sm.avg_row / nzl));
     const int passes = (int)std::clamp(std::ceil(kItemIters / iters), 1.0, (double)kMaxPasses);

@@ -20,7 +20,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMergeItems = 2048;
-constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, workgroups dealt round-robin
 
 template <int L, class V>
 __device__ __forceinline__ V group_sum(V v) {
@@ -40,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void csr_vector_kernel(int64_t m, const int
                                                             const V* __restrict__ x, V* __restrict__ y, V alpha,
                                                             V beta) {
   const int lane = threadIdx.x & (L - 1);
-  const int64_t row = (int64_t)blockIdx.x * (kBlock / L) + threadIdx.x / L;
+  const int64_t row = xcd_block(blockIdx.x, gridDim.x) * (kBlock / L) + threadIdx.x / L;
   if (row >= m) return;  // whole L-lane groups leave together
   const int s = rp[row], e = rp[row + 1];
   V sum = 0, sum2 = 0;
@@ -192,7 +191,8 @@ __global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int*
                                                            V beta, const int* __restrict__ coord,
                                                            int* __restrict__ carry_row, V* __restrict__ carry_val,
                                                            V* __restrict__ head_val) {
-  merge_tile<V, false>(blockIdx.x, m, rp, ci, val, x, y, alpha, beta, coord, carry_row, carry_val, head_val);
+  merge_tile<V, false>((int)xcd_block(blockIdx.x, gridDim.x), m, rp, ci, val, x, y, alpha, beta, coord, carry_row,
+                       carry_val, head_val);
 }
 
 // Column panels: the matrix split into P column ranges (P a multiple of the 8

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void sliced_spmv_kernel(int64_t m, int64_t 
                                                              const int* __restrict__ col, const V* __restrict__ sv,
                                                              const V* __restrict__ x, V* __restrict__ y,
                                                              V* __restrict__ partial, V alpha, V beta) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int64_t u = t / C;
   if (u >= nunits) return;
   const int r = (int)(t - u * C);
