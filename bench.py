@@ -12,8 +12,11 @@ One step = one y = A*x with the matrix and vectors resident in HBM.
 Prints ONE JSON line. `value` is the effective bandwidth (algorithmic bytes:
 values + indices + x read once + y written) of the fastest kernel (least time)
 on the FP64 lognormal matrix, divided by the wall time per step. `serving`
-holds the batched workload (sspmv_amd/bin/serve_bench): requests/s and p50 /
-p99 latency of the packed persistent kernel against plain per-call CSR. `roofline` compares the
+(256 resident matrices, k uniform in 1..32, saturation and Poisson arrivals)
+and `batched` (1,024 matrices, k in {1, 4, 8, 16, 32}, saturation) come from
+sspmv_amd/bin/serve_bench: requests/s and p50 / p99 latency of the packed
+persistent kernel against per-call CSR (the plain kernel and the server's own
+row engine, synchronous and asynchronous). `roofline` compares the
 same bytes with that kernel's device time (hipEvents on its stream) against
 MI355X's 8 TB/s. `cpu_baseline` times the CPU oracle's kernel_flag=2 CSR SpMV
 (the reference's load-balanced OpenMP path, restated) on this host's cores.
@@ -109,12 +112,15 @@ def cpu_baseline(A, x, threads, max_seconds):
     return ms, iters, y
 
 
-def serving(matrices, requests, rates):
-    """The batched serving workload (sspmv_amd/bin/serve_bench): 256 resident
-    matrices, requests with k in 1..32, packed persistent kernel vs per-call CSR."""
+def serving(matrices, requests, rates, ks=None):
+    """The batched serving workload (sspmv_amd/bin/serve_bench): resident
+    matrices (256 for serving, 1,024 for the batched launch), requests with k in
+    1..32 (or the given list), packed persistent kernel vs per-call CSR."""
     import subprocess
     exe = os.path.join(ROOT, "sspmv_amd", "bin", "serve_bench")
     cmd = [exe, f"--matrices={matrices}", f"--requests={requests}", f"--rates={rates}"]
+    if ks:
+        cmd.append(f"--ks={ks}")
     log("[serving] " + " ".join(cmd[1:]))
     out = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
     for line in out.stderr.splitlines():
@@ -125,13 +131,16 @@ def serving(matrices, requests, rates):
     sat = r["server_saturation"]["req_per_s"]
     r["speedup_vs_percall_async"] = round(sat / r["percall_async"]["req_per_s"], 3)
     r["speedup_vs_percall_sync"] = round(sat / r["percall_sync"]["req_per_s"], 3)
+    r["speedup_vs_percall_engine_async"] = round(sat / r["percall_engine_async"]["req_per_s"], 3)
+    r["speedup_vs_percall_engine_sync"] = round(sat / r["percall_engine_sync"]["req_per_s"], 3)
     return r
 
 
 def spmm_section(lib, steps):
     """Multi-vector products on a 4x4-block random matrix (1M rows, 8 blocks per
     block row, the batched workload's block family at full size): BSR 4x4 on
-    the matrix cores, BSR scalar, CSR; GFLOP/s and effective GB/s."""
+    the matrix cores, BSR scalar, the plain CSR kernel and the CSR row engine
+    (4, 8, 16 lanes per row); GFLOP/s and effective GB/s."""
     A = lib.generate(lib.GEN_BLOCKRAND, [262144, 262144, 4, 4, 8], 11)
     M = lib.Matrix.from_host(A)
     M.convert(lib.BSR, lib.make_params(block_rows=4, block_cols=4))
@@ -142,7 +151,8 @@ def spmm_section(lib, steps):
         row = {}
         for label, fmt, var in (("BSR-4x4-mfma", lib.BSR, lib.VARIANT_AUTO), ("BSR-4x4-scalar", lib.BSR,
                                                                               lib.VARIANT_VECTOR),
-                                ("CSR", lib.CSR, lib.VARIANT_AUTO)):
+                                ("CSR", lib.CSR, lib.VARIANT_AUTO), ("CSR-engine-4", lib.CSR, 18),
+                                ("CSR-engine-8", lib.CSR, 19), ("CSR-engine-16", lib.CSR, 20)):
             M.time_spmm(fmt, k, 1.0, X, k, 0.0, Y, k, 3, var)
             ms = M.time_spmm(fmt, k, 1.0, X, k, 0.0, Y, k, steps, var)
             eff = M.effective_bytes(fmt, k, False)
@@ -170,7 +180,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bound on the CPU baseline's timed work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--formats", default="all", help="comma list of kernel labels to measure, or 'all'")
-    ap.add_argument("--sections", default="lognormal,lognormal_fp32,stencil27,spmm,copy,serving")
+    ap.add_argument("--sections", default="lognormal,lognormal_fp32,stencil27,spmm,copy,serving,batched")
     ap.add_argument("--serve-matrices", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--serve-rates", default="5000,20000,50000")
@@ -279,6 +289,8 @@ def main():
         out["spmm_blockrand4x4_1M"] = spmm_section(lib, 20)
     if "serving" in sections:
         out["serving"] = serving(args.serve_matrices, args.serve_requests, args.serve_rates)
+    if "batched" in sections:
+        out["batched"] = serving(1024, 4096, "", "1,4,8,16,32")
     print(json.dumps(out), flush=True)
 
 

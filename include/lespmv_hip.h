@@ -142,7 +142,12 @@ int lespmv_convert_ms(lespmv_matrix_t A, int fmt, double* ms);
 int lespmv_spmv(lespmv_matrix_t A, int fmt, int variant, double alpha, const void* x, double beta, void* y,
                 lespmv_stream_t stream);
 /* Y = alpha*A*X + beta*Y for k right-hand sides, X row-major N x k (leading
- * dimension ldx >= k), Y row-major M x k (ldy >= k), device pointers. */
+ * dimension ldx >= k), Y row-major M x k (ldy >= k), device pointers.
+ * CSR: variant 16 + log2(L) (L = 1..64 lanes per row) runs the row engine of
+ * the serving kernel (several right-hand sides per lane, eight gathers in
+ * flight per lane; bitwise the result the server gives for the same request),
+ * any other variant the plain kernel (a group of k lanes per row, one
+ * right-hand side per lane). */
 int lespmv_spmm(lespmv_matrix_t A, int fmt, int variant, int32_t k, double alpha, const void* X, int64_t ldx,
                 double beta, void* Y, int64_t ldy, lespmv_stream_t stream);
 

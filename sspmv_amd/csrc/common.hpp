@@ -264,21 +264,15 @@ inline int64_t align_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 // workgroups to the XCDs round-robin, so workgroup b runs on XCD b % 8.
 constexpr int kXcds = 8;
 
-#ifndef LSP_XCD_REMAP
-#define LSP_XCD_REMAP 1
-#endif
 // Logical block of hardware workgroup b in a grid of nb: each XCD takes one
 // contiguous eighth of the logical blocks, so the rows one XCD works on (and
 // the x lines they share with their neighbours) stay in that XCD's L2 instead
-// of being fetched by all eight. A bijection on [0, nb).
+// of being fetched by all eight. A bijection on [0, nb). Only for layouts
+// whose work per block does not trend along the rows (a layout sorted by row
+// length would put all long rows on one XCD).
 __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
-#if LSP_XCD_REMAP
   const int64_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, i = b / kXcds;
   return x * q + (x < r ? x : r) + i;
-#else
-  (void)nb;
-  return b;
-#endif
 }
 
 // Grid size helper (1D, capped to the hardware limit).

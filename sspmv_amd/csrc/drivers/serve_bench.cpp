@@ -4,17 +4,19 @@
 // drawn from 2-D / 3-D stencils, lognormal-row random matrices and 4x4-block
 // random matrices with 8 blocks per block row; FP64), makes them resident in a
 // lespmv server (lanes per row autotuned per matrix), and draws a request
-// stream: matrix uniform, k uniform in {1..32}. It then measures
+// stream: matrix uniform, k uniform over --ks (default 1..32). It then measures
 //   * the server at saturation (every request queued at once) and under
 //     Poisson arrivals at the given rates: requests/s and p50 / p99 latency
 //     (submit to completion, host clock);
-//   * plain per-call CSR on the same requests: lespmv_spmm with the default
-//     CSR kernel, (a) one synchronous call per request (the reference's
-//     LeSpMV_csr usage), (b) all calls issued asynchronously on the matrices'
-//     own streams and one synchronisation at the end (its throughput bound).
+//   * per-call CSR on the same requests: lespmv_spmm with the plain CSR SpMM
+//     kernel and with the server's own row engine (same lanes per matrix),
+//     (a) one synchronous call per request (the reference's LeSpMV_csr
+//     usage), (b) all calls issued asynchronously on the matrices' own
+//     streams and one synchronisation at the end (its throughput bound).
 // Prints one JSON object on stdout, progress on stderr.
 // Options: --matrices=N --requests=N --rates=r1,r2,... --seed=S --budget-gb=G --slots=N
 //          --families=0123 (0 2-D Laplacian, 1 27-point stencil, 2 lognormal rows, 3 4x4 blocks)
+//          --ks=k1,k2,... (right-hand-side counts, each in 1..32)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -66,7 +68,7 @@ static double mean(const std::vector<double>& v) {
 }
 
 struct Mat {
-  int32_t id = -1;
+  int32_t id = -1, lanes = 1;
   lespmv_matrix_t h = nullptr;
   int64_t m = 0, n = 0, nnz = 0;
   void *X = nullptr, *Y = nullptr, *Y2 = nullptr;
@@ -84,6 +86,24 @@ int main(int argc, char** argv) {
   const uint64_t seed = std::stoull(arg(argc, argv, "--seed", "1000"));
   const double budget_gb = std::stod(arg(argc, argv, "--budget-gb", "200"));
   const int nslots = std::stoi(arg(argc, argv, "--slots", "4"));
+  std::vector<int> ks;  // right-hand-side counts to draw from (default 1..32)
+  {
+    const std::string r = arg(argc, argv, "--ks", "");
+    size_t p = 0;
+    while (p < r.size()) {
+      size_t q = r.find(',', p);
+      if (q == std::string::npos) q = r.size();
+      ks.push_back(std::stoi(r.substr(p, q - p)));
+      p = q + 1;
+    }
+    if (ks.empty())
+      for (int k = 1; k <= 32; k++) ks.push_back(k);
+    for (int k : ks)
+      if (k < 1 || k > 32) {
+        std::fprintf(stderr, "--ks values must lie in 1..32\n");
+        return 2;
+      }
+  }
   std::vector<int> families;  // generator families to draw from (default all four)
   {
     const std::string f = arg(argc, argv, "--families", "0,1,2,3");
@@ -140,6 +160,7 @@ int main(int argc, char** argv) {
     M.nnz = c.nnz;
     M.fam = fam;
     CHECK(lespmv_server_add_csr(S, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, 0, &M.id));
+    CHECK(lespmv_server_matrix(S, M.id, nullptr, &M.lanes));
     CHECK(lespmv_create_csr(&M.h, LESPMV_F64, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, 0));
     lespmv_host_csr_free(&c);
     CHECK(lespmv_malloc(&M.X, 8 * M.n * 32));
@@ -162,7 +183,7 @@ int main(int argc, char** argv) {
   for (int r = 0; r < nreq; r++) {
     const uint64_t h = mix(seed * 104729 + (uint64_t)r);
     reqs[r].mat = (int)(h % (uint64_t)nmat);
-    reqs[r].k = 1 + (int)(mix(h) % 32);
+    reqs[r].k = ks[mix(h) % ks.size()];
   }
   // flops, and the bytes a request must move at least: its CSR matrix
   // (row pointers, indices, values), X once and Y once
@@ -214,13 +235,21 @@ int main(int argc, char** argv) {
     return std::string(buf);
   };
 
-  auto run_percall_sync = [&]() {
+  // per-call kernels: the plain CSR SpMM (variant AUTO), or the server's own
+  // row engine with the matrix's tuned lanes (variant 16 + log2 lanes)
+  auto variant_of = [&](const Mat& M, bool engine) {
+    if (!engine) return (int)LESPMV_VARIANT_AUTO;
+    int v = 16;
+    while ((1 << (v - 16)) < M.lanes) v++;
+    return v;
+  };
+  auto run_percall_sync = [&](bool engine) {
     std::vector<double> lat;
     const double t0 = now_us();
     for (const auto& q : reqs) {
       Mat& M = mats[q.mat];
       const double a = now_us();
-      CHECK(lespmv_spmm(M.h, LESPMV_CSR, LESPMV_VARIANT_AUTO, q.k, 1.0, M.X, q.k, 0.0, M.Y, q.k, nullptr));
+      CHECK(lespmv_spmm(M.h, LESPMV_CSR, variant_of(M, engine), q.k, 1.0, M.X, q.k, 0.0, M.Y, q.k, nullptr));
       CHECK(lespmv_synchronize());
       lat.push_back(now_us() - a);
     }
@@ -229,16 +258,16 @@ int main(int argc, char** argv) {
     std::snprintf(buf, sizeof buf,
                   "{\"req_per_s\": %.1f, \"p50_us\": %.1f, \"p99_us\": %.1f, \"gflops\": %.1f, \"GBs\": %.1f}",
                   nreq / secs, pct(lat, 0.5), pct(lat, 0.99), flops / secs * 1e-9, bytes / secs * 1e-9);
-    std::fprintf(stderr, "per-call sync %s\n", buf);
+    std::fprintf(stderr, "per-call %s sync %s\n", engine ? "engine" : "plain", buf);
     return std::string(buf);
   };
 
-  auto run_percall_async = [&]() {
+  auto run_percall_async = [&](bool engine) {
     const double t0 = now_us();
     for (size_t r = 0; r < reqs.size(); r++) {
       const Req& q = reqs[r];
       Mat& M = mats[q.mat];
-      CHECK(lespmv_spmm(M.h, LESPMV_CSR, LESPMV_VARIANT_AUTO, q.k, 1.0, M.X, q.k, 0.0, (r & 1) ? M.Y2 : M.Y, q.k,
+      CHECK(lespmv_spmm(M.h, LESPMV_CSR, variant_of(M, engine), q.k, 1.0, M.X, q.k, 0.0, (r & 1) ? M.Y2 : M.Y, q.k,
                         nullptr));
     }
     CHECK(lespmv_synchronize());
@@ -246,7 +275,7 @@ int main(int argc, char** argv) {
     char buf[256];
     std::snprintf(buf, sizeof buf, "{\"req_per_s\": %.1f, \"gflops\": %.1f, \"GBs\": %.1f}", nreq / secs,
                   flops / secs * 1e-9, bytes / secs * 1e-9);
-    std::fprintf(stderr, "per-call async %s\n", buf);
+    std::fprintf(stderr, "per-call %s async %s\n", engine ? "engine" : "plain", buf);
     return std::string(buf);
   };
 
@@ -254,6 +283,8 @@ int main(int argc, char** argv) {
   for (int i = 0; i < nmat; i++) {
     CHECK(lespmv_server_submit(S, mats[i].id, 4, 1.0, mats[i].X, 4, 0.0, mats[i].Y, 4, nullptr));
     CHECK(lespmv_spmm(mats[i].h, LESPMV_CSR, LESPMV_VARIANT_AUTO, 4, 1.0, mats[i].X, 4, 0.0, mats[i].Y2, 4, nullptr));
+    CHECK(lespmv_spmm(mats[i].h, LESPMV_CSR, variant_of(mats[i], true), 4, 1.0, mats[i].X, 4, 0.0, mats[i].Y2, 4,
+                      nullptr));
   }
   CHECK(lespmv_server_drain(S));
   CHECK(lespmv_synchronize());
@@ -293,8 +324,10 @@ int main(int argc, char** argv) {
   const std::string sat = run_server(0.0);
   std::vector<std::string> loads;
   for (double r : rates) loads.push_back(run_server(r));
-  const std::string sync = run_percall_sync();
-  const std::string async = run_percall_async();
+  const std::string sync = run_percall_sync(false);
+  const std::string async = run_percall_async(false);
+  const std::string esync = run_percall_sync(true);
+  const std::string easync = run_percall_async(true);
   CHECK(lespmv_server_stats(S, &st));
 
   std::printf("{\"matrices\": %d, \"requests\": %d, \"bytes_per_request\": %.0f, \"total_rows\": %lld, \"total_nnz\": %lld, "
@@ -303,8 +336,9 @@ int main(int argc, char** argv) {
               nmat, nreq, bytes / nreq, (long long)total_rows, (long long)total_nnz, (long long)st.bytes_used, build_s,
               (long long)st.batches, sat.c_str());
   for (size_t i = 0; i < loads.size(); i++) std::printf("%s%s", i ? ", " : "", loads[i].c_str());
-  std::printf("], \"unloaded\": %s, \"percall_sync\": %s, \"percall_async\": %s}\n", unloaded.c_str(),
-              sync.c_str(), async.c_str());
+  std::printf("], \"unloaded\": %s, \"percall_sync\": %s, \"percall_async\": %s, \"percall_engine_sync\": %s, "
+              "\"percall_engine_async\": %s}\n",
+              unloaded.c_str(), sync.c_str(), async.c_str(), esync.c_str(), easync.c_str());
   for (auto& M : mats) {
     lespmv_free(M.X);
     lespmv_free(M.Y);

@@ -8,35 +8,37 @@
 // X, Y, k, alpha, beta, rows per work item, first work item); a request's work
 // items are consecutive row ranges, sized on the host from the matrix's mean
 // row length and k so that an item is a few passes of ~kItemIters loads per
-// lane. The persistent kernel's waves take a few items at a time (the batch's
-// grab size, about eight grabs per wave) from an atomic counter, find the request by a binary search over the first item numbers
-// held in LDS, and compute the rows for all k right-hand sides: a row gets G
-// lanes (G = max(lanes, k rounded up to a power of two)), lane t handles
-// right-hand side t mod kk and every (G/kk)-th nonzero, and the partial sums
-// meet in xor shuffles. The lanes a row gets depend only on its matrix and k,
-// never on the batch, so a request's result is bitwise the same alone and in
-// any mix.
+// lane. Each request is placed on one XCD (largest work first, least loaded
+// XCD): its items form that XCD's queue, so its X is gathered into one L2.
+// The persistent kernel's waves take a few items at a time (the batch's grab
+// size, about eight grabs per wave) from their own XCD's counter and then help
+// the other queues, find the request by a search over the first item numbers
+// held in LDS, and compute the rows for all k right-hand sides: a lane covers
+// W right-hand sides (1, 2 or 4, one vector access when aligned), a row gets G
+// lanes (G = max(lanes, kk), kk the lanes that cover k), lane t handles
+// right-hand sides (t mod kk)*W.. and every (G/kk)-th nonzero, and the partial
+// sums meet in xor shuffles. W, G and the order of every sum depend only on
+// the request, never on the batch or on which wave runs an item, so a
+// request's result is bitwise the same alone and in any mix.
 //
-// Each in-flight slot owns a stream, pinned host table, device table, counter
-// and a captured graph: copy the table in, reset the counter with a one-thread
-// kernel (no memset node), run the persistent kernel.
+// Each in-flight slot owns a stream, pinned host table, device table, per-XCD
+// counters and captured graphs: copy the table in (resized to the batch),
+// reset the counters with a one-wave kernel (no memset node), run the
+// persistent kernel.
 #include <algorithm>
 #include <chrono>
 #include <deque>
 #include <vector>
 
 #include "common.hpp"
+#include "rows_engine.hpp"
 
 namespace lsp {
 
 namespace {
 
 constexpr int kMaxBatch = 256;
-constexpr int kItemIters = 32;  // target loads per lane in one work item
-constexpr int kMaxPasses = 8;   // passes of 64 / G rows in one work item
 constexpr int kServeBlock = 256;
-constexpr int kChains = 8;  // nonzeros per lane whose loads are issued together
-constexpr int kModeW[5] = {1, 2, 2, 4, 4};  // right-hand sides per lane of each request mode
 
 struct alignas(16) ReqDesc {
   const int* rp;
@@ -51,122 +53,27 @@ struct alignas(16) ReqDesc {
   int32_t mode, pad[3];  // right-hand sides per lane and vector access, see rhs_mode
 };
 
+// Each request is placed on one XCD: its work items form that XCD's queue
+// [xbegin[x], xbegin[x + 1]) with its own counter, so the request's X is
+// gathered into one L2 instead of eight.
 struct alignas(16) BatchHeader {
   int32_t nreq, total_items, grab, pad;  // grab: work items a wave takes per atomic
+  int32_t xbegin[kXcds + 1];
+  int32_t pad2[3];
 };
 
 constexpr size_t kTableBytes = sizeof(BatchHeader) + sizeof(ReqDesc) * kMaxBatch;
+constexpr int kCounterStride = 64;  // ints between the per-XCD counters (one 256-byte line each)
 
 __global__ void reset_counter(int* c) {
-  if (threadIdx.x == 0) *c = 0;
+  if (threadIdx.x < kXcds) c[threadIdx.x * kCounterStride] = 0;
 }
 
-// Pointers taken from a request descriptor are generic to the compiler; in the
-// global address space it issues global (not flat) loads, which wait on the
-// vector-memory counter alone.
-#define GLOBAL_AS __attribute__((address_space(1)))
-
-// X and Y values of one lane's W right-hand sides: consecutive in memory; one
-// vector access when the request is aligned for it (VEC), else W guarded
-// scalar accesses.
-template <class V, int W, bool VEC>
-__device__ __forceinline__ void load_w(const GLOBAL_AS V* __restrict__ p, int nvalid, V* out) {
-  if constexpr (VEC && W > 1) {
-    typedef V vec_t __attribute__((ext_vector_type(W)));
-    const vec_t v = *(const GLOBAL_AS vec_t*)p;
-#pragma unroll
-    for (int u = 0; u < W; u++) out[u] = v[u];
-  } else {
-#pragma unroll
-    for (int u = 0; u < W; u++) out[u] = u < nvalid ? p[u] : (V)0;
-  }
-}
-
-// Rows [r0, r1) of request q for all k right-hand sides. A row gets G lanes
-// (G = max(lanes, kk), kk = lanes needed to cover k at W right-hand sides
-// each, rounded up to a power of two); lane t handles right-hand sides
-// (t mod kk)*W .. +W-1 and every (G/kk)-th nonzero of the row, and the
-// partial sums meet in xor shuffles. A lane issues the index, value and x
-// loads of kChains nonzeros before it uses any of them (out-of-row slots are
-// masked), so that many gathers are in flight per lane.
-template <class V, int W, bool VEC>
-__device__ __forceinline__ void run_rows(const ReqDesc& q, int r0, int r1, int lane) {
-  constexpr int CH = kChains;
-  const int k = q.k;
-  const int kw = (k + W - 1) / W;
-  int kk = 1;
-  while (kk < kw) kk <<= 1;
-  const int G = q.lanes > kk ? q.lanes : kk;  // lanes per row (power of two <= 64)
-  const int P = 64 / G;                       // rows per pass
-  const int grp = lane / G, t = lane % G, j = (t % kk) * W, s = t / kk, nzl = G / kk;
-  const int nvalid = k - j;
-  const GLOBAL_AS V* __restrict__ val = (const GLOBAL_AS V*)q.val;
-  const GLOBAL_AS V* __restrict__ X = (const GLOBAL_AS V*)q.X + j;
-  GLOBAL_AS V* __restrict__ Y = (GLOBAL_AS V*)q.Y;
-  const GLOBAL_AS int* __restrict__ rp = (const GLOBAL_AS int*)q.rp;
-  const GLOBAL_AS int* __restrict__ ci = (const GLOBAL_AS int*)q.ci;
-  const int64_t ldx = q.ldx;
-  const V alpha = (V)q.alpha, beta = (V)q.beta;
-  for (int base = r0; base < r1; base += P) {
-    const int row = base + grp;
-    const bool ok = row < r1 && j < k;
-    V acc[W];
-#pragma unroll
-    for (int u = 0; u < W; u++) acc[u] = 0;
-    if (ok) {
-      const int e = rp[row + 1];
-      for (int p = rp[row] + s; p < e; p += CH * nzl) {
-        int cc[CH];
-        V vv[CH], xx[CH][W];
-#pragma unroll
-        for (int c = 0; c < CH; c++) {
-          const int pc = p + c * nzl;
-          cc[c] = pc < e ? ci[pc] : 0;
-          vv[c] = pc < e ? val[pc] : (V)0;
-        }
-#pragma unroll
-        for (int c = 0; c < CH; c++) {
-          if (p + c * nzl < e) {
-            load_w<V, W, VEC>(X + (int64_t)cc[c] * ldx, nvalid, xx[c]);
-          } else {
-#pragma unroll
-            for (int u = 0; u < W; u++) xx[c][u] = 0;
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < CH; c++)
-#pragma unroll
-          for (int u = 0; u < W; u++) acc[u] += vv[c] * xx[c][u];
-      }
-    }
-    V sum[W];
-#pragma unroll
-    for (int u = 0; u < W; u++) {
-      V a = acc[u];
-      for (int off = kk; off < G; off <<= 1) a += __shfl_xor(a, off);
-      sum[u] = a;
-    }
-    if (ok && s == 0) {
-      GLOBAL_AS V* y = Y + (int64_t)row * q.ldy + j;
-      if constexpr (VEC && W > 1) {
-        typedef V vec_t __attribute__((ext_vector_type(W)));
-        vec_t o;
-        if (beta == (V)0) {
-#pragma unroll
-          for (int u = 0; u < W; u++) o[u] = alpha * sum[u];
-        } else {
-          const vec_t old = *(const GLOBAL_AS vec_t*)y;
-#pragma unroll
-          for (int u = 0; u < W; u++) o[u] = alpha * sum[u] + beta * old[u];
-        }
-        *(GLOBAL_AS vec_t*)y = o;
-      } else {
-#pragma unroll
-        for (int u = 0; u < W; u++)
-          if (u < nvalid) y[u] = beta == (V)0 ? alpha * sum[u] : alpha * sum[u] + beta * y[u];
-      }
-    }
-  }
+// The XCD this wave runs on (hardware register XCC_ID).
+__device__ __forceinline__ int xcc_id() {
+  int id;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(id));
+  return id & (kXcds - 1);
 }
 
 template <class V>
@@ -180,12 +87,20 @@ __global__ __launch_bounds__(kServeBlock) void packed_kernel(const unsigned char
   if (threadIdx.x == 0) s_begin[nreq] = total;
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  for (;;) {
-    int first = 0;
-    if (lane == 0) first = atomicAdd(counter, grab);
-    first = __builtin_amdgcn_readfirstlane(first);  // lane 0's value, wave-uniform from here on
-    if (first >= total) break;  // every wave ends here once the items are taken
-    const int last = first + grab < total ? first + grab : total;
+  const int home = xcc_id();
+  // the queue of the wave's own XCD first, then help the others; a queue is
+  // left once its counter passes its end, so every wave ends after at most
+  // one failed atomic per queue
+  for (int v = 0; v < kXcds; v++) {
+    const int xq = (home + v) & (kXcds - 1);
+    const int qbeg = H->xbegin[xq], qend = H->xbegin[xq + 1];
+    for (;;) {
+      if (qbeg >= qend) break;
+      int first = 0;
+      if (lane == 0) first = atomicAdd(counter + xq * kCounterStride, grab);
+      first = __builtin_amdgcn_readfirstlane(first) + qbeg;  // lane 0's value, wave-uniform from here on
+      if (first >= qend) break;
+      const int last = first + grab < qend ? first + grab : qend;
     int lo = 0, hi = nreq - 1;  // last request whose first item <= first
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -205,6 +120,7 @@ __global__ __launch_bounds__(kServeBlock) void packed_kernel(const unsigned char
         case 3: run_rows<V, 4, false>(q, r0, r1, lane); break;
         default: run_rows<V, 4, true>(q, r0, r1, lane); break;
       }
+    }
     }
   }
 }
@@ -281,7 +197,8 @@ struct Server {
     LSP_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_table), kTableBytes, hipHostMallocDefault));
     std::memset(sl.h_table, 0, kTableBytes);
     sl.d_table.alloc(kTableBytes);
-    sl.counter.alloc(sizeof(int));
+    sl.counter.alloc(sizeof(int) * kCounterStride * kXcds);
+    LSP_HIP(hipMemset(sl.counter.p, 0, sizeof(int) * kCounterStride * kXcds));
     for (int g = 0; g < kGraphs; g++) {
       LSP_HIP(hipStreamBeginCapture(sl.s, hipStreamCaptureModeThreadLocal));
       LSP_HIP(hipMemcpyAsync(sl.d_table.p, sl.h_table, kTableBytes, hipMemcpyHostToDevice, sl.s));
@@ -334,19 +251,42 @@ struct Server {
     queue.erase(queue.begin(), queue.begin() + n);
     auto* H = reinterpret_cast<BatchHeader*>(sl.h_table);
     auto* R = reinterpret_cast<ReqDesc*>(sl.h_table + sizeof(BatchHeader));
-    int32_t item = 0;
+    // each request on one XCD: largest estimated work (lane steps) first, on
+    // the least loaded XCD; descriptors grouped by XCD
+    std::vector<int> mode(n), xcd(n), order(n);
+    std::vector<double> work(n);
     for (int i = 0; i < n; i++) {
       const Pending& p = sl.reqs[i];
-      ServerMat& sm = mats[p.mat];
-      Matrix& A = *sm.A;
-      const int mode = rhs_mode(p);
-      const int rpi = rows_per_item(sm, p.k, kModeW[mode]);
-      const int32_t nitems = (int32_t)ceil_div(A.m, rpi);
-      R[i] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, p.X, p.Y, p.ldx, p.ldy, p.alpha, p.beta,
-                     (int32_t)A.m, p.k, sm.lanes, rpi, item, nitems, mode, {0, 0, 0}};
-      item += nitems;
-      sm.last_use = ++tick;
+      const Matrix& A = *mats[p.mat].A;
+      mode[i] = rhs_mode(p);
+      work[i] = (double)A.m + (double)A.nnz * (double)ceil_div(p.k, kModeW[mode[i]]);
+      order[i] = i;
     }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    double load[kXcds] = {};
+    for (int i : order) {
+      const int x = (int)(std::min_element(load, load + kXcds) - load);
+      xcd[i] = x;
+      load[x] += work[i];
+    }
+    int32_t item = 0;
+    int d = 0;
+    for (int x = 0; x < kXcds; x++) {
+      H->xbegin[x] = item;
+      for (int i = 0; i < n; i++) {
+        if (xcd[i] != x) continue;
+        const Pending& p = sl.reqs[i];
+        ServerMat& sm = mats[p.mat];
+        Matrix& A = *sm.A;
+        const int rpi = rows_per_item(sm, p.k, kModeW[mode[i]]);
+        const int32_t nitems = (int32_t)ceil_div(A.m, rpi);
+        R[d++] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, p.X, p.Y, p.ldx, p.ldy, p.alpha, p.beta,
+                         (int32_t)A.m, p.k, sm.lanes, rpi, item, nitems, mode[i], {0, 0, 0}};
+        item += nitems;
+        sm.last_use = ++tick;
+      }
+    }
+    H->xbegin[kXcds] = item;
     H->nreq = n;
     H->total_items = item;
     // the smallest grid with at least one wave per item (a short batch finishes
@@ -366,29 +306,9 @@ struct Server {
     items += item;
   }
 
-  // right-hand sides per lane (W) and whether a lane's W values move as one
-  // vector: that needs k, ldx and ldy multiples of W and X, Y aligned to the
-  // vector. Depends on the request alone, never on the batch.
-  int rhs_mode(const Pending& p) const {
-    const int W = p.k == 1 ? 1 : p.k <= 3 ? 2 : 4;
-    if (W == 1) return 0;
-    const uintptr_t align = (uintptr_t)W * (uintptr_t)vsize(dtype);
-    const bool vec = p.k % W == 0 && p.ldx % W == 0 && p.ldy % W == 0 &&
-                     reinterpret_cast<uintptr_t>(p.X) % align == 0 && reinterpret_cast<uintptr_t>(p.Y) % align == 0;
-    return W == 2 ? (vec ? 2 : 1) : (vec ? 4 : 3);
-  }
+  int rhs_mode(const Pending& p) const { return rows_mode(p.k, p.ldx, p.ldy, p.X, p.Y, vsize(dtype)); }
 
-  // rows in one work item: passes of 64 / G rows, enough passes for about
-  // kItemIters loads per lane at the matrix's mean row length
-  static int rows_per_item(const ServerMat& sm, int k, int W) {
-    const int kw = (k + W - 1) / W;
-    int kk = 1;
-    while (kk < kw) kk <<= 1;
-    const int G = std::max(sm.lanes, kk), nzl = G / kk;
-    const double iters = std::max(1.0, std::ceil(sm.avg_row / nzl));
-    const int passes = (int)std::clamp(std::ceil(kItemIters / iters), 1.0, (double)kMaxPasses);
-    return passes * (64 / G);
-  }
+  static int rows_per_item(const ServerMat& sm, int k, int W) { return rows_item(sm.avg_row, sm.lanes, k, W); }
 
   // retire finished slots; launch queued requests on free slots
   void progress(bool wait) {

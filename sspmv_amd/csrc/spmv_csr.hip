@@ -11,9 +11,12 @@
 //    workgroup gets the same amount of work whatever the row lengths. The tile's
 //    products are staged in LDS, complete rows are reduced from LDS, and the row
 //    open at a tile edge is carried to a small deterministic fix-up kernel.
+// SpMM has a plain kernel (a group of k lanes per row) and the row engine of
+// rows_engine.hpp (variants 16 + log2 L).
 #include <algorithm>
 
 #include "common.hpp"
+#include "rows_engine.hpp"
 
 namespace lsp {
 namespace {
@@ -440,6 +443,45 @@ void launch_spmm(Matrix& A, int k, double alpha, const void* X, int64_t ldx, dou
   LSP_LAUNCHED();
 }
 
+// SpMM through the row engine: one work item (rpi rows) per wave.
+struct RowsArgs {
+  const int* rp;
+  const int* ci;
+  const void* val;
+  const void* X;
+  void* Y;
+  int64_t ldx, ldy;
+  double alpha, beta;
+  int32_t m, k, lanes, rpi;
+};
+
+template <class V, int W, bool VEC>
+__global__ __launch_bounds__(kBlock) void csr_rows_kernel(RowsArgs a, int64_t nitems) {
+  const int64_t item = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + threadIdx.x / 64;
+  if (item >= nitems) return;  // whole waves leave together
+  const int r0 = (int)(item * a.rpi);
+  const int r1 = r0 + a.rpi < a.m ? r0 + a.rpi : a.m;
+  run_rows<V, W, VEC>(a, r0, r1, threadIdx.x & 63);
+}
+
+template <class V>
+void rows_dispatch(Matrix& A, int lanes, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
+                   int64_t ldy, hipStream_t s) {
+  const int mode = rows_mode(k, ldx, ldy, X, Y, (int64_t)sizeof(V));
+  const int rpi = rows_item(A.meta.avg_row, lanes, k, kModeW[mode]);
+  const int64_t nitems = ceil_div(A.m, rpi);
+  const RowsArgs a{A.rp.as<int>(), A.ci.as<int>(), A.val.p, X, Y, ldx, ldy, alpha, beta, (int32_t)A.m, k, lanes, rpi};
+  const dim3 grid = grid1d(nitems, kBlock / 64);
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((csr_rows_kernel<V, 1, false>), grid, dim3(kBlock), 0, s, a, nitems); break;
+    case 1: hipLaunchKernelGGL((csr_rows_kernel<V, 2, false>), grid, dim3(kBlock), 0, s, a, nitems); break;
+    case 2: hipLaunchKernelGGL((csr_rows_kernel<V, 2, true>), grid, dim3(kBlock), 0, s, a, nitems); break;
+    case 3: hipLaunchKernelGGL((csr_rows_kernel<V, 4, false>), grid, dim3(kBlock), 0, s, a, nitems); break;
+    default: hipLaunchKernelGGL((csr_rows_kernel<V, 4, true>), grid, dim3(kBlock), 0, s, a, nitems); break;
+  }
+  LSP_LAUNCHED();
+}
+
 template <class V>
 void spmm_dispatch(Matrix& A, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
                    hipStream_t s) {
@@ -512,8 +554,15 @@ void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, 
 
 void csr_spmm(Matrix& A, int variant, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
               int64_t ldy, hipStream_t s) {
-  (void)variant;
   if (A.m == 0 || k == 0) return;
+  if (variant >= 16 && variant <= 22) {  // the row engine with 2^(variant - 16) lanes per row
+    LSP_REQUIRE(A.m <= INT32_MAX, LESPMV_ERR_UNSUPPORTED, "the row engine takes M < 2^31");
+    if (!A.meta.analysed) csr_analyse(A, s);
+    const int lanes = 1 << (variant - 16);
+    if (A.dtype == LESPMV_F64) rows_dispatch<double>(A, lanes, k, alpha, X, ldx, beta, Y, ldy, s);
+    else rows_dispatch<float>(A, lanes, k, alpha, X, ldx, beta, Y, ldy, s);
+    return;
+  }
   if (A.dtype == LESPMV_F64) spmm_dispatch<double>(A, k, alpha, X, ldx, beta, Y, ldy, s);
   else spmm_dispatch<float>(A, k, alpha, X, ldx, beta, Y, ldy, s);
 }

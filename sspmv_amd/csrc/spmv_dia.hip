@@ -65,7 +65,7 @@ template <class V>
 __global__ __launch_bounds__(kBlock) void dia_vector(int64_t m, int64_t n, int nd, int64_t stride,
                                                      const int64_t* __restrict__ offs, const V* __restrict__ data,
                                                      const V* __restrict__ x, V* __restrict__ y, V alpha, V beta) {
-  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= m) return;
   V sum = 0;
   for (int d = 0; d < nd; d++) {
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void dia_window(int64_t m, int64_t n, int n
                                                      const int* __restrict__ cwidth, const int* __restrict__ cbase) {
   extern __shared__ unsigned char smem[];
   V* win = reinterpret_cast<V*>(smem);
-  const int64_t r0 = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+  const int64_t r0 = (int64_t)blockIdx.x * kBlock;
   for (int c = 0; c < nclusters; c++) {
     const int64_t start = r0 + clo[c];
     const int w = cwidth[c], base = cbase[c];

@@ -86,8 +86,10 @@ __global__ __launch_bounds__(kBlock) void sliced_spmv_kernel(int64_t m, int64_t 
                                                              const int* __restrict__ order,
                                                              const int* __restrict__ col, const V* __restrict__ sv,
                                                              const V* __restrict__ x, V* __restrict__ y,
-                                                             V* __restrict__ partial, V alpha, V beta) {
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+                                                             V* __restrict__ partial, V alpha, V beta,
+                                                             int remap) {
+  const int64_t blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  const int64_t t = blk * kBlock + threadIdx.x;
   const int64_t u = t / C;
   if (u >= nunits) return;
   const int r = (int)(t - u * C);
@@ -337,7 +339,7 @@ void sliced_run(Matrix& A, SlicedUnits& d, double alpha, const void* x, double b
   hipLaunchKernelGGL((sliced_spmv_kernel<V>), grid1d(d.nunits * d.C, kBlock), dim3(kBlock), 0, s, A.m, d.nunits, d.C,
                      d.units.as<int4>(), d.chunk_off.as<int64_t>(), d.reorder.as<int>(), d.col.as<int>(),
                      d.val.as<V>(), static_cast<const V*>(x), static_cast<V*>(y), d.partial.as<V>(), (V)alpha,
-                     (V)beta);
+                     (V)beta, (int)((int64_t)d.sigma * kXcds * 8 <= A.m));  // XCD-contiguous rows unless sorted globally
   LSP_LAUNCHED();
   if (d.nsplit) {
     hipLaunchKernelGGL((sliced_fixup<V>), grid1d(d.nsplit * d.C, kBlock), dim3(kBlock), 0, s, A.m, d.nsplit, d.C,

@@ -120,3 +120,35 @@ def test_csr_spmm_columns_equal_spmv(gpu, oracle, k):
     for j in range(k):
         ref = oracle_y(oracle, A, 0.8, X[:, j].copy(), 0.7, Y0[:, j].copy())
         assert rel2(Y[:, j], ref) <= 1e-12, j
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_csr_spmm_row_engine(gpu, oracle, dt):
+    """The row engine (SpMM variants 16 + log2 L, the serving kernel's rows):
+    every column against the oracle's SpMV, for k giving one, two and four
+    right-hand sides per lane, with vector access (leading dimension k) and
+    with scalar access (leading dimension k + 1, whose extra column must stay
+    untouched)."""
+    rng = np.random.default_rng(5)
+    mats = [random_csr(3000, 2500, seed=21, mean=9.0, long_rows=3, long_len=900, dtype=dt),
+            suite(dt)["rand_empty"], suite(dt)["single_row"], suite(dt)["rect_tall"]]
+    worst = 0.0
+    for A in mats:
+        M = gpu.Matrix.from_host(A)
+        for k in (1, 2, 3, 4, 5, 8, 17, 32, 64):
+            for extra in (0, 1):
+                ld = k + extra
+                X = rng.uniform(-1, 1, (A.n, ld)).astype(dt)
+                Y0 = rng.uniform(-1, 1, (A.m, ld)).astype(dt)
+                refs = [oracle_y(oracle, A, 0.8, X[:, j].copy(), 0.7, Y0[:, j].copy()) for j in range(k)]
+                for var in (16, 18, 22):
+                    dX, dY = gpu.DeviceArray.from_numpy(X), gpu.DeviceArray.from_numpy(Y0)
+                    M.spmm(CSR, k, 0.8, dX, ld, 0.7, dY, ld, variant=var)
+                    Y = dY.numpy()
+                    for j in range(k):
+                        err = rel2(Y[:, j], refs[j])
+                        worst = max(worst, err)
+                        assert err <= TOL[np.dtype(dt)], (A.m, k, extra, var, j, err)
+                    if extra:
+                        assert np.array_equal(Y[:, k:], Y0[:, k:])
+    print(f"row engine {np.dtype(dt).name}: worst rel2 {worst:.2e}")

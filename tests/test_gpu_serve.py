@@ -54,6 +54,11 @@ def test_packed_kernel_matches_percall_and_oracle(gpu, oracle):
         dX, dY = lib.DeviceArray.from_numpy(X), lib.DeviceArray.from_numpy(Y0)
         M.spmm(lib.CSR, k, alpha, dX, k, beta, dY, k)
         assert rel2(Y, dY.numpy()) <= 1e-12, (i, k)
+        # the row engine launched on its own (same lanes) gives bitwise the server's result
+        lanes = S.lanes(mids[i])
+        dY2 = lib.DeviceArray.from_numpy(Y0)
+        M.spmm(lib.CSR, k, alpha, dX, k, beta, dY2, k, variant=16 + lanes.bit_length() - 1)
+        np.testing.assert_array_equal(Y, dY2.numpy())
         # column 0 against the oracle's CSR SpMV
         ref = oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v, alpha, np.ascontiguousarray(X[:, 0]), beta,
                               np.ascontiguousarray(Y0[:, 0]))
