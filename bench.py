@@ -18,8 +18,10 @@ sspmv_amd/bin/serve_bench: requests/s and p50 / p99 latency of the packed
 persistent kernel against per-call CSR (the plain kernel and the server's own
 row engine, synchronous and asynchronous). `roofline` compares the
 same bytes with that kernel's device time (hipEvents on its stream) against
-MI355X's 8 TB/s. `cpu_baseline` times the CPU oracle's kernel_flag=2 CSR SpMV
-(the reference's load-balanced OpenMP path, restated) on this host's cores.
+MI355X's 8 TB/s. `cpu_baseline` times the reference's load-balanced OpenMP
+CSR SpMV (kernel_flag=2), built from the reference's sources into oracle/_ref
+(or, where that build is absent, the oracle's restatement of it) on this
+host's cores.
 """
 from __future__ import annotations
 
@@ -46,7 +48,8 @@ def kernels(lib, section):
     """(label, fmt, variant, params) measured per section."""
     sell = lambda C, s: lib.make_params(chunk=C, sigma=s, alignment=1)
     if section == "stencil27":
-        return [("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None), ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0,
+        return [("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None), ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None),
+                ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0,
                                                                     sell(64, 4096)),
                 ("DIA", lib.DIA, lib.VARIANT_VECTOR, lib.make_params(alignment=1, max_diags=10240)),
                 ("DIA-lds", lib.DIA, lib.VARIANT_MERGE, lib.make_params(alignment=1, max_diags=10240)),
@@ -100,16 +103,56 @@ def summary(results):
             for r in results}
 
 
+def reference_csr_bench():
+    """The reference's own LeSpMV_csr (kernel_flag 2) built by oracle/build_ref.sh
+    into oracle/_ref/, at the instruction level this host runs (x86-64-v4 with
+    AVX-512, else x86-64-v3); None when it was not built."""
+    import ctypes as C
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = f.read()
+    except OSError:
+        flags = ""
+    v4 = all(t in flags for t in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"))
+    level = "x86-64-v4" if v4 else "x86-64-v3"
+    path = os.path.join(ROOT, "oracle", "_ref", f"libref_csr_bench_{level}.so")
+    if not os.path.exists(path):
+        return None, level
+    lib = C.CDLL(path)
+    p = np.ctypeslib.ndpointer
+    lib.refbench_csr_lb.argtypes = [C.c_int, C.c_int, C.c_int, p(np.int32), p(np.int32), p(np.float64),
+                                    p(np.float64), p(np.float64), C.c_int, C.c_int]
+    lib.refbench_csr_lb.restype = C.c_double
+    return lib, level
+
+
 def cpu_baseline(A, x, threads, max_seconds):
-    from oracle.oracle import Oracle
-    O = Oracle()
+    """ms per CPU SpMV, iterations, y, kind and a description of the kernel:
+    the reference's own code where oracle/_ref holds it, else the oracle's
+    restatement of the same kernel."""
+    ref, level = reference_csr_bench()
+    rp = np.ascontiguousarray(A.rp, dtype=np.int32)
+    ci = np.ascontiguousarray(A.ci, dtype=np.int32)
+    v = np.ascontiguousarray(A.v, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.zeros(A.m)
+    if ref is not None:
+        def time(iters):
+            return ref.refbench_csr_lb(A.m, A.n, A.nnz, rp, ci, v, x, y, threads, iters)
+        kind, what = "reference", f"the reference's LeSpMV_csr kernel_flag=2 built from its sources ({level})"
+    else:
+        from oracle.oracle import Oracle
+        O = Oracle()
+
+        def time(iters):
+            return O.time_csr_lb(A.m, A.n, rp, ci, v, x, y, threads, iters)
+        kind, what = "port", "kernel_flag=2 restated, OpenMP"
     # reference iteration policy (include/spmv_benchmark.h:185-195), bounded by max_seconds
-    t_one = O.time_csr_lb(A.m, A.n, A.rp, A.ci, A.v, x, y, threads, 1)
+    t_one = time(1)
     iters = 1000 if t_one < 20.0 else int(min(1000, max(20, 20000.0 / t_one)))
     iters = max(1, min(iters, int(max_seconds * 1000.0 / max(t_one, 1e-3))))
-    ms = O.time_csr_lb(A.m, A.n, A.rp, A.ci, A.v, x, y, threads, iters)
-    return ms, iters, y
+    ms = time(iters)
+    return ms, iters, y, kind, what
 
 
 def serving(matrices, requests, rates, ks=None):
@@ -259,14 +302,13 @@ def main():
         out["copy_GBs"] = round(gbs, 1)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ms, iters, y_cpu = cpu_baseline(A, x, threads, args.cpu_seconds)
+        ms, iters, y_cpu, kind, what = cpu_baseline(A, x, threads, args.cpu_seconds)
         out["cpu_baseline"] = {
             "value": round(M.effective_bytes(lib.CSR) / (ms * 1e-3) / 1e9, 3),
             "unit": "GB/s",
             "cores": threads,
-            "kind": "port",
-            "sample": f"the full lognormal matrix, {iters} CSR SpMVs (kernel_flag=2 restated, OpenMP), "
-                      f"{ms:.2f} ms each",
+            "kind": kind,
+            "sample": f"the full lognormal matrix, {iters} CSR SpMVs ({what}, OpenMP), {ms:.2f} ms each",
         }
         nb = np.linalg.norm(y_cpu)
         out["parity_rel2_vs_cpu"] = float(np.linalg.norm(y_gpu - y_cpu) / (nb if nb else 1.0))

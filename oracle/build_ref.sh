@@ -27,3 +27,11 @@ SRCS=(
   "$HERE/ref_driver.cpp" "$HERE/ref_features.cpp" "${SRCS[@]}" \
   -Wl,-rpath,/opt/rocm/lib/llvm/lib -o "$OUT/libref_lespmv.so"
 echo "built $OUT/libref_lespmv.so"
+# the CPU baseline (ref_bench.cpp over the reference's CSR kernel), in two
+# portable instruction levels instead of -march=native
+for LEVEL in x86-64-v3 x86-64-v4; do
+  "$CXX" -O3 -std=c++17 -fPIC -shared -fopenmp -march=$LEVEL -w \
+    -I"$REF/include" "$HERE/ref_bench.cpp" "$REF/src/spmv_csr.cpp" "$REF/utils/thread.cpp" "$REF/utils/partition.cpp" \
+    -Wl,-rpath,/opt/rocm/lib/llvm/lib -o "$OUT/libref_csr_bench_$LEVEL.so"
+  echo "built $OUT/libref_csr_bench_$LEVEL.so"
+done
