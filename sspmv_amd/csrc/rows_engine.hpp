@@ -14,7 +14,7 @@
 
 namespace lsp {
 
-constexpr int kItemIters = 32;  // target loads per lane in one work item
+constexpr int kItemIters = 64;  // target loads per lane in one work item
 constexpr int kMaxPasses = 8;   // passes of 64 / G rows in one work item
 constexpr int kChains = 8;      // nonzeros per lane whose loads are issued together
 constexpr int kModeW[5] = {1, 2, 2, 4, 4};  // right-hand sides per lane of each mode

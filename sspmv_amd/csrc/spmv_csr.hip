@@ -142,19 +142,35 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
     if (i < nz) prod[i] = v[q] * xv[q];
   }
   __syncthreads();
-  // rows whose end lies in this tile: short ones by one thread each
-  for (int i = tid; i < r1 - r0; i += kBlock) {
-    const int s = rows[i], e = rows[i + 1];
-    const int sl = (s > z0 ? s : z0) - z0, el = e - z0;
-    if (el - sl > kLongRow) {
-      long_rows[atomicAdd(&nlong, 1)] = i;
-      continue;
-    }
+  // rows whose end lies in this tile: short ones by rl threads each (thread
+  // q of the group adds entries q, q + rl, ...; xor shuffles combine the
+  // group); rl = 4 when the tile's rows average 16 entries or more, else 1
+  // (tiles of column panels hold rows of one or two entries). The loop bound
+  // is uniform, so every lane takes part in the shuffles.
+  const int nrows = r1 - r0;
+  const int rl = nz >= 16 * nrows ? 4 : 1;
+  for (int base = 0; base < nrows; base += kBlock / rl) {
+    const int i = base + tid / rl, q = tid % rl;
     V sum = 0;
-    for (int k = sl; k < el; ++k) sum += prod[k];
-    const int r = r0 + i;
-    if (s >= z0) y[r] = axpby(alpha, sum, beta, y + r);
-    else head_val[b] = sum;  // i == 0: row started in an earlier tile
+    bool mine = false;
+    int s = 0;
+    if (i < nrows) {
+      s = rows[i];
+      const int e = rows[i + 1];
+      const int sl = (s > z0 ? s : z0) - z0, el = e - z0;
+      if (el - sl > kLongRow) {
+        if (q == 0) long_rows[atomicAdd(&nlong, 1)] = i;
+      } else {
+        for (int k = sl + q; k < el; k += rl) sum += prod[k];
+        mine = true;
+      }
+    }
+    for (int off = 1; off < rl; off <<= 1) sum += __shfl_xor(sum, off);
+    if (mine && q == 0) {
+      const int r = r0 + i;
+      if (s >= z0) y[r] = axpby(alpha, sum, beta, y + r);
+      else head_val[b] = sum;  // i == 0: row started in an earlier tile
+    }
   }
   __syncthreads();
   // long rows: one wavefront each

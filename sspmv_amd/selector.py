@@ -287,6 +287,26 @@ def run_training(db_path, model_out, report_out=None, epochs=400, seed=7, tie=0.
     return mlp, res
 
 
+def run_evaluation(db_path, model_path, report_out=None, log=print):
+    """A trained model on a timing database it was not trained on (for
+    instance one collected with a different seed, or after kernel changes):
+    the same regret figures as the training report, all matrices held out."""
+    db = load_db(db_path)
+    X, times = db["X"].astype(np.float32), db["times"]
+    mlp = lib.Mlp(path=model_path)
+    res = {
+        "matrices": int(len(X)), "model": os.path.basename(model_path),
+        "selector": evaluate(times, picks(mlp, X, db["allowed"])),
+        "default_csr": evaluate(times, np.full(len(X), N_GRID)),
+        "rule_dia_else_csr": evaluate(times, rule_pick(db)),
+    }
+    if report_out:
+        with open(report_out, "w") as f:
+            json.dump(res, f, indent=1)
+    log(json.dumps(res), flush=True)
+    return res
+
+
 class Selector:
     """A trained selector (weights file written by ``run_training``)."""
 
@@ -314,10 +334,16 @@ def main(argv=None):
     t.add_argument("--db-out", help="where to write the merged database")
     t.add_argument("--epochs", type=int, default=400)
     t.add_argument("--tie", type=float, default=0.0)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--db", required=True)
+    e.add_argument("--model", required=True)
+    e.add_argument("--report")
     a = ap.parse_args(argv)
     lib.check(lib.load().lespmv_init(0))
     if a.cmd == "collect":
         collect(a.count, a.out, a.seed, a.iters, a.max_nnz, (1e4, a.n_max))
+    elif a.cmd == "evaluate":
+        run_evaluation(a.db, a.model, a.report)
     else:
         path = a.db[0]
         if len(a.db) > 1 or a.db_out:  # merge several databases
