@@ -152,3 +152,21 @@ def test_csr_spmm_row_engine(gpu, oracle, dt):
                     if extra:
                         assert np.array_equal(Y[:, k:], Y0[:, k:])
     print(f"row engine {np.dtype(dt).name}: worst rel2 {worst:.2e}")
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_csr_merge_wide_rows(gpu, oracle, dt):
+    """Merge-path tiles whose rows average 16+ entries add each row with four
+    threads (27-point stencil, 4x4 blocks with 8 blocks per block row) and
+    shorter ones with one (5-point Laplacian): all against the oracle."""
+    rng = np.random.default_rng(7)
+    mats = [gpu.generate(gpu.GEN_STENCIL27, [22, 26.0, -1.0], 3, dt),
+            gpu.generate(gpu.GEN_BLOCKRAND, [3000, 2900, 4, 4, 8], 4, dt),
+            gpu.generate(gpu.GEN_LAPLACE2D, [97, 61, 4.0, -1.0], 5, dt)]
+    for A in mats:
+        M = gpu.Matrix.from_host(A)
+        x = rng.uniform(-1, 1, A.n).astype(dt)
+        y0 = rng.uniform(-1, 1, A.m).astype(dt)
+        ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+        got = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, gpu.VARIANT_MERGE)
+        assert rel2(got, ref) <= TOL[np.dtype(dt)], (A.m, A.nnz)
