@@ -21,9 +21,9 @@
 // the request, never on the batch or on which wave runs an item, so a
 // request's result is bitwise the same alone and in any mix.
 //
-// Each in-flight slot owns a stream, pinned host table, device table, per-XCD
-// counters and captured graphs: copy the table in (resized to the batch),
-// reset the counters with a one-wave kernel (no memset node), run the
+// Each in-flight slot owns a stream, pinned host table, device table (per-XCD
+// counters, header, descriptors) and captured graphs: copy the table in
+// (resized to the batch; the copy also zeroes the counters), run the
 // persistent kernel.
 #include <algorithm>
 #include <chrono>
@@ -62,12 +62,13 @@ struct alignas(16) BatchHeader {
   int32_t pad2[3];
 };
 
-constexpr size_t kTableBytes = sizeof(BatchHeader) + sizeof(ReqDesc) * kMaxBatch;
-constexpr int kCounterStride = 64;  // ints between the per-XCD counters (one 256-byte line each)
-
-__global__ void reset_counter(int* c) {
-  if (threadIdx.x < kXcds) c[threadIdx.x * kCounterStride] = 0;
-}
+// Device table = the per-XCD counters (one 256-byte line each), the header,
+// the descriptors. The host copy keeps its counters at zero, so the table
+// copy that starts every launch also resets them (no memset node, no reset
+// kernel).
+constexpr int kCounterStride = 64;  // ints between the per-XCD counters
+constexpr size_t kCounterBytes = sizeof(int) * kCounterStride * kXcds;
+constexpr size_t kTableBytes = kCounterBytes + sizeof(BatchHeader) + sizeof(ReqDesc) * kMaxBatch;
 
 // The XCD this wave runs on (hardware register XCC_ID).
 __device__ __forceinline__ int xcc_id() {
@@ -160,7 +161,7 @@ struct Slot {
   hipGraphExec_t exec[kGraphs] = {};
   hipGraphNode_t copy[kGraphs] = {};  // the table copy of each graph, resized per launch
   unsigned char* h_table = nullptr;
-  DBuf d_table, counter;
+  DBuf d_table;
   std::vector<Pending> reqs;
   bool busy = false;
 };
@@ -197,18 +198,15 @@ struct Server {
     LSP_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_table), kTableBytes, hipHostMallocDefault));
     std::memset(sl.h_table, 0, kTableBytes);
     sl.d_table.alloc(kTableBytes);
-    sl.counter.alloc(sizeof(int) * kCounterStride * kXcds);
-    LSP_HIP(hipMemset(sl.counter.p, 0, sizeof(int) * kCounterStride * kXcds));
     for (int g = 0; g < kGraphs; g++) {
       LSP_HIP(hipStreamBeginCapture(sl.s, hipStreamCaptureModeThreadLocal));
       LSP_HIP(hipMemcpyAsync(sl.d_table.p, sl.h_table, kTableBytes, hipMemcpyHostToDevice, sl.s));
-      hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, sl.s, sl.counter.as<int>());
       if (dtype == LESPMV_F64)
         hipLaunchKernelGGL(packed_kernel<double>, dim3(grids[g]), dim3(kServeBlock), 0, sl.s,
-                           sl.d_table.as<unsigned char>(), sl.counter.as<int>());
+                           sl.d_table.as<unsigned char>() + kCounterBytes, sl.d_table.as<int>());
       else
         hipLaunchKernelGGL(packed_kernel<float>, dim3(grids[g]), dim3(kServeBlock), 0, sl.s,
-                           sl.d_table.as<unsigned char>(), sl.counter.as<int>());
+                           sl.d_table.as<unsigned char>() + kCounterBytes, sl.d_table.as<int>());
       LSP_HIP(hipStreamEndCapture(sl.s, &sl.graph[g]));
       size_t nn = 0;
       LSP_HIP(hipGraphGetNodes(sl.graph[g], nullptr, &nn));
@@ -249,8 +247,8 @@ struct Server {
     const int n = (int)std::min<size_t>(queue.size(), (size_t)max_batch);
     sl.reqs.assign(queue.begin(), queue.begin() + n);
     queue.erase(queue.begin(), queue.begin() + n);
-    auto* H = reinterpret_cast<BatchHeader*>(sl.h_table);
-    auto* R = reinterpret_cast<ReqDesc*>(sl.h_table + sizeof(BatchHeader));
+    auto* H = reinterpret_cast<BatchHeader*>(sl.h_table + kCounterBytes);
+    auto* R = reinterpret_cast<ReqDesc*>(sl.h_table + kCounterBytes + sizeof(BatchHeader));
     // each request on one XCD: largest estimated work (lane steps) first, on
     // the least loaded XCD; descriptors grouped by XCD
     std::vector<int> mode(n), xcd(n), order(n);
@@ -297,7 +295,7 @@ struct Server {
     H->grab = (int32_t)std::clamp<int64_t>(item / ((int64_t)grids[g] * (kServeBlock / 64) * 8), 1, 64);
     // copy only the descriptors in use (the slot's previous launch has finished)
     LSP_HIP(hipGraphExecMemcpyNodeSetParams1D(sl.exec[g], sl.copy[g], sl.d_table.p, sl.h_table,
-                                              sizeof(BatchHeader) + sizeof(ReqDesc) * (size_t)n,
+                                              kCounterBytes + sizeof(BatchHeader) + sizeof(ReqDesc) * (size_t)n,
                                               hipMemcpyHostToDevice));
     LSP_HIP(hipGraphLaunch(sl.exec[g], sl.s));
     LSP_HIP(hipEventRecord(sl.done, sl.s));

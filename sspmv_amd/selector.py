@@ -261,16 +261,18 @@ def split(n, frac=0.8, seed=3):
     return idx[:k], idx[k:]
 
 
-def run_training(db_path, model_out, report_out=None, epochs=400, seed=7, tie=0.0, log=print):
+def run_training(db_path, model_out, report_out=None, epochs=400, seed=7, tie=0.0, weight_decay=1e-4, hidden=128,
+                 log=print):
     db = load_db(db_path)
     X, times = db["X"].astype(np.float32), db["times"]
     L = labels(times, tie)
     tr, te = split(len(X))
-    mlp, hist = train(X[tr], L[tr], epochs=epochs, seed=seed)
+    mlp, hist = train(X[tr], L[tr], epochs=epochs, seed=seed, weight_decay=weight_decay, hidden=hidden)
     mlp.save(model_out)
     cand = picks(mlp, X[te], db["allowed"][te])
     res = {
         "matrices": int(len(X)), "train": int(len(tr)), "test": int(len(te)), "epochs": epochs, "tie": tie,
+        "weight_decay": weight_decay, "hidden": hidden,
         "final_loss": float(hist[-1]), "first_loss": float(hist[0]),
         "selector": evaluate(times[te], cand),
         "default_csr": evaluate(times[te], np.full(len(te), N_GRID)),
@@ -334,6 +336,8 @@ def main(argv=None):
     t.add_argument("--db-out", help="where to write the merged database")
     t.add_argument("--epochs", type=int, default=400)
     t.add_argument("--tie", type=float, default=0.0)
+    t.add_argument("--weight-decay", type=float, default=1e-4)
+    t.add_argument("--hidden", type=int, default=128)
     e = sub.add_parser("evaluate")
     e.add_argument("--db", required=True)
     e.add_argument("--model", required=True)
@@ -351,7 +355,7 @@ def main(argv=None):
             merged = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
             path = a.db_out or a.model + ".db.npz"
             np.savez_compressed(path, **merged)
-        run_training(path, a.model, a.report, a.epochs, tie=a.tie)
+        run_training(path, a.model, a.report, a.epochs, tie=a.tie, weight_decay=a.weight_decay, hidden=a.hidden)
 
 
 if __name__ == "__main__":
