@@ -33,14 +33,14 @@ inline int rows_mode(int k, int64_t ldx, int64_t ldy, const void* X, const void*
 }
 
 // Rows in one work item: passes of 64 / G rows, enough passes for about
-// kItemIters loads per lane at the matrix's mean row length.
-inline int rows_item(double avg_row, int lanes, int k, int W) {
+// `iters` loads per lane at the matrix's mean row length.
+inline int rows_item(double avg_row, int lanes, int k, int W, int iters = kItemIters) {
   const int kw = (k + W - 1) / W;
   int kk = 1;
   while (kk < kw) kk <<= 1;
   const int G = std::max(lanes, kk), nzl = G / kk;
-  const double iters = std::max(1.0, std::ceil(avg_row / nzl));
-  const int passes = (int)std::clamp(std::ceil(kItemIters / iters), 1.0, (double)kMaxPasses);
+  const double per_pass = std::max(1.0, std::ceil(avg_row / nzl));
+  const int passes = (int)std::clamp(std::ceil(iters / per_pass), 1.0, (double)kMaxPasses);
   return passes * (64 / G);
 }
 

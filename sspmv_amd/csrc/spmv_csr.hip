@@ -459,7 +459,10 @@ void launch_spmm(Matrix& A, int k, double alpha, const void* X, int64_t ldx, dou
   LSP_LAUNCHED();
 }
 
-// SpMM through the row engine: one work item (rpi rows) per wave.
+// SpMM through the row engine: one work item (rpi rows) per wave. A launch
+// of its own wants more, smaller items than a packed batch (which already
+// fills the GPU with many requests).
+constexpr int kCallIters = 16;
 struct RowsArgs {
   const int* rp;
   const int* ci;
@@ -484,7 +487,7 @@ template <class V>
 void rows_dispatch(Matrix& A, int lanes, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
                    int64_t ldy, hipStream_t s) {
   const int mode = rows_mode(k, ldx, ldy, X, Y, (int64_t)sizeof(V));
-  const int rpi = rows_item(A.meta.avg_row, lanes, k, kModeW[mode]);
+  const int rpi = rows_item(A.meta.avg_row, lanes, k, kModeW[mode], kCallIters);
   const int64_t nitems = ceil_div(A.m, rpi);
   const RowsArgs a{A.rp.as<int>(), A.ci.as<int>(), A.val.p, X, Y, ldx, ldy, alpha, beta, (int32_t)A.m, k, lanes, rpi};
   const dim3 grid = grid1d(nitems, kBlock / 64);
