@@ -462,7 +462,7 @@ void launch_spmm(Matrix& A, int k, double alpha, const void* X, int64_t ldx, dou
 // SpMM through the row engine: one work item (rpi rows) per wave. A launch
 // of its own wants more, smaller items than a packed batch (which already
 // fills the GPU with many requests).
-constexpr int kCallIters = 16;
+constexpr int kCallIters = 8;
 struct RowsArgs {
   const int* rp;
   const int* ci;
