@@ -244,9 +244,20 @@ struct Server {
   }
 
   void launch(Slot& sl) {
-    const int n = (int)std::min<size_t>(queue.size(), (size_t)max_batch);
-    sl.reqs.assign(queue.begin(), queue.begin() + n);
-    queue.erase(queue.begin(), queue.begin() + n);
+    // the longest queue prefix (up to max_batch) whose work items fit in
+    // int32; it leaves the queue only once its table is built (an error while
+    // building leaves every request queued)
+    int n = 0;
+    int64_t total = 0;
+    while (n < (int)std::min<size_t>(queue.size(), (size_t)max_batch)) {
+      const Pending& p = queue[n];
+      const int64_t m = mats[p.mat].A->m;
+      const int64_t add = ceil_div(m, rows_per_item(mats[p.mat], p.k, kModeW[rhs_mode(p)]));
+      if (n > 0 && total + add >= INT32_MAX) break;
+      total += add;
+      n++;
+    }
+    std::vector<Pending> batch(queue.begin(), queue.begin() + n);
     auto* H = reinterpret_cast<BatchHeader*>(sl.h_table + kCounterBytes);
     auto* R = reinterpret_cast<ReqDesc*>(sl.h_table + kCounterBytes + sizeof(BatchHeader));
     // each request on one XCD: largest estimated work (lane steps) first, on
@@ -254,7 +265,7 @@ struct Server {
     std::vector<int> mode(n), xcd(n), order(n);
     std::vector<double> work(n);
     for (int i = 0; i < n; i++) {
-      const Pending& p = sl.reqs[i];
+      const Pending& p = batch[i];
       const Matrix& A = *mats[p.mat].A;
       mode[i] = rhs_mode(p);
       work[i] = (double)A.m + (double)A.nnz * (double)ceil_div(p.k, kModeW[mode[i]]);
@@ -273,7 +284,7 @@ struct Server {
       H->xbegin[x] = item;
       for (int i = 0; i < n; i++) {
         if (xcd[i] != x) continue;
-        const Pending& p = sl.reqs[i];
+        const Pending& p = batch[i];
         ServerMat& sm = mats[p.mat];
         Matrix& A = *sm.A;
         const int rpi = rows_per_item(sm, p.k, kModeW[mode[i]]);
@@ -281,7 +292,6 @@ struct Server {
         R[d++] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, p.X, p.Y, p.ldx, p.ldy, p.alpha, p.beta,
                          (int32_t)A.m, p.k, sm.lanes, rpi, item, nitems, mode[i], {0, 0, 0}};
         item += nitems;
-        sm.last_use = ++tick;
       }
     }
     H->xbegin[kXcds] = item;
@@ -297,6 +307,9 @@ struct Server {
     LSP_HIP(hipGraphExecMemcpyNodeSetParams1D(sl.exec[g], sl.copy[g], sl.d_table.p, sl.h_table,
                                               kCounterBytes + sizeof(BatchHeader) + sizeof(ReqDesc) * (size_t)n,
                                               hipMemcpyHostToDevice));
+    for (const Pending& p : batch) mats[p.mat].last_use = ++tick;
+    queue.erase(queue.begin(), queue.begin() + n);
+    sl.reqs = std::move(batch);
     LSP_HIP(hipGraphLaunch(sl.exec[g], sl.s));
     LSP_HIP(hipEventRecord(sl.done, sl.s));
     sl.busy = true;
