@@ -47,13 +47,24 @@ __global__ __launch_bounds__(kBlock) void csr_vector_kernel(int64_t m, const int
   const int s = rp[row], e = rp[row + 1];
   V sum = 0, sum2 = 0;
   int j = s + lane;
-  for (; j + 3 * L < e; j += 4 * L) {
+  // whole rounds of 4L entries (the same count for every lane of the row),
+  // then the rest of the row in one round with its loads masked past the end:
+  // short rows get all their loads in flight at once
+  for (; j - lane + 4 * L <= e; j += 4 * L) {
     const int c0 = ci[j], c1 = ci[j + L], c2 = ci[j + 2 * L], c3 = ci[j + 3 * L];
     const V v0 = val[j], v1 = val[j + L], v2 = val[j + 2 * L], v3 = val[j + 3 * L];
     sum += v0 * x[c0] + v1 * x[c1];
     sum2 += v2 * x[c2] + v3 * x[c3];
   }
-  for (; j < e; j += L) sum += val[j] * x[ci[j]];
+  if (j < e) {
+    const bool b1 = j + L < e, b2 = j + 2 * L < e, b3 = j + 3 * L < e;
+    const int c0 = ci[j], c1 = b1 ? ci[j + L] : 0, c2 = b2 ? ci[j + 2 * L] : 0, c3 = b3 ? ci[j + 3 * L] : 0;
+    const V v0 = val[j], v1 = b1 ? val[j + L] : V(0), v2 = b2 ? val[j + 2 * L] : V(0);
+    const V v3 = b3 ? val[j + 3 * L] : V(0);
+    const V x0 = x[c0], x1 = b1 ? x[c1] : V(0), x2 = b2 ? x[c2] : V(0), x3 = b3 ? x[c3] : V(0);
+    sum += v0 * x0 + v1 * x1;
+    sum2 += v2 * x2 + v3 * x3;
+  }
   sum = group_sum<L>(sum + sum2);
   if (lane == 0) y[row] = axpby(alpha, sum, beta, y + row);
 }
