@@ -280,12 +280,17 @@ typedef struct {
 } lespmv_server_stats_t;
 int lespmv_server_create(lespmv_server_t* out, int32_t dtype, int64_t hbm_budget, int32_t max_batch, int32_t nslots);
 int lespmv_server_destroy(lespmv_server_t S);
-/* Copies a host CSR in; lanes = rows-per-wave setting (power of two <= 64), 0 = autotune. */
+/* Copies a host CSR in; lanes = lanes per row (rows per wave = 64 / lanes; power of two <= 64) for every
+ * request, or 0 = autotune: SpMV lanes from the CSR vector candidates, SpMM lanes from timing the row
+ * engine at k = 2 and k = 16 (requests with k = 2..3 and k >= 4 use those). */
 int lespmv_server_add_csr(lespmv_server_t S, int64_t m, int64_t n, int64_t nnz, const int32_t* row_offset,
                           const int32_t* col_index, const void* values, int32_t lanes, int32_t* id);
 int lespmv_server_remove(lespmv_server_t S, int32_t id);
-/* The resident matrix as a handle (owned by the server) and its lanes setting. */
+/* The resident matrix as a handle (owned by the server) and its lanes setting for k = 1. */
 int lespmv_server_matrix(lespmv_server_t S, int32_t id, lespmv_matrix_t* A, int32_t* lanes);
+/* Lanes per row the server uses for this matrix's requests with k right-hand sides (the same product
+ * through lespmv_spmm with CSR variant 16 + log2(lanes) gives bitwise the server's result). */
+int lespmv_server_lanes(lespmv_server_t S, int32_t id, int32_t k, int32_t* lanes);
 int lespmv_server_submit(lespmv_server_t S, int32_t id, int32_t k, double alpha, const void* X, int64_t ldx,
                          double beta, void* Y, int64_t ldy, int64_t* request_id);
 /* Completed requests since the last poll (wait = 1 blocks until at least one

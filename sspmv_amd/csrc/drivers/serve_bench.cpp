@@ -17,6 +17,7 @@
 // Options: --matrices=N --requests=N --rates=r1,r2,... --seed=S --budget-gb=G --slots=N
 //          --families=0123 (0 2-D Laplacian, 1 27-point stencil, 2 lognormal rows, 3 4x4 blocks)
 //          --ks=k1,k2,... (right-hand-side counts, each in 1..32)
+//          --lanes=L (lanes per row for every matrix instead of the server's tuning)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -86,6 +87,7 @@ int main(int argc, char** argv) {
   const uint64_t seed = std::stoull(arg(argc, argv, "--seed", "1000"));
   const double budget_gb = std::stod(arg(argc, argv, "--budget-gb", "200"));
   const int nslots = std::stoi(arg(argc, argv, "--slots", "4"));
+  const int fixed_lanes = std::stoi(arg(argc, argv, "--lanes", "0"));  // 0: the server tunes lanes per matrix
   std::vector<int> ks;  // right-hand-side counts to draw from (default 1..32)
   {
     const std::string r = arg(argc, argv, "--ks", "");
@@ -159,7 +161,7 @@ int main(int argc, char** argv) {
     M.n = c.n;
     M.nnz = c.nnz;
     M.fam = fam;
-    CHECK(lespmv_server_add_csr(S, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, 0, &M.id));
+    CHECK(lespmv_server_add_csr(S, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, fixed_lanes, &M.id));
     CHECK(lespmv_server_matrix(S, M.id, nullptr, &M.lanes));
     CHECK(lespmv_create_csr(&M.h, LESPMV_F64, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, 0));
     lespmv_host_csr_free(&c);

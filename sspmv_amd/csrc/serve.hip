@@ -136,7 +136,8 @@ struct ServerMat {
   std::unique_ptr<Matrix> A;
   lespmv_matrix_t h = nullptr;  // the same object, for the C API
   double avg_row = 0.0;
-  int32_t lanes = 1;
+  int32_t lanes = 1;                   // k = 1
+  int32_t lanes_w[3] = {1, 1, 1};      // per right-hand sides per lane W = 1, 2, 4
   int64_t bytes = 0, last_use = 0, pending = 0;
   bool live = false;
 };
@@ -290,7 +291,7 @@ struct Server {
         const int rpi = rows_per_item(sm, p.k, kModeW[mode[i]]);
         const int32_t nitems = (int32_t)ceil_div(A.m, rpi);
         R[d++] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, p.X, p.Y, p.ldx, p.ldy, p.alpha, p.beta,
-                         (int32_t)A.m, p.k, sm.lanes, rpi, item, nitems, mode[i], {0, 0, 0}};
+                         (int32_t)A.m, p.k, lanes_of(sm, kModeW[mode[i]]), rpi, item, nitems, mode[i], {0, 0, 0}};
         item += nitems;
       }
     }
@@ -319,7 +320,11 @@ struct Server {
 
   int rhs_mode(const Pending& p) const { return rows_mode(p.k, p.ldx, p.ldy, p.X, p.Y, vsize(dtype)); }
 
-  static int rows_per_item(const ServerMat& sm, int k, int W) { return rows_item(sm.avg_row, sm.lanes, k, W); }
+  static int lanes_of(const ServerMat& sm, int W) { return sm.lanes_w[W == 1 ? 0 : W == 2 ? 1 : 2]; }
+
+  static int rows_per_item(const ServerMat& sm, int k, int W) {
+    return rows_item(sm.avg_row, lanes_of(sm, W), k, W);
+  }
 
   // retire finished slots; launch queued requests on free slots
   void progress(bool wait) {
@@ -356,6 +361,32 @@ namespace {
 Server& srv(lespmv_server_t h) {
   LSP_REQUIRE(h, LESPMV_ERR_ARG, "null server");
   return *reinterpret_cast<Server*>(h);
+}
+
+// SpMM lanes per row: the row engine timed through lespmv_spmm at k = 2
+// (W = 2) and k = 16 (W = 4) for 1..16 lanes; the fastest wins.
+void tune_spmm_lanes(Server& S, ServerMat& sm, lespmv_matrix_t mh, int64_t n) {
+  if (sm.A->nnz == 0) return;
+  const int64_t vs = vsize(S.dtype);
+  DBuf X, Y;
+  X.alloc(vs * n * 16);
+  Y.alloc(vs * sm.A->m * 16);
+  LSP_HIP(hipMemset(X.p, 0, vs * n * 16));
+  const int ks[2] = {2, 16};
+  for (int c = 1; c < 3; c++) {
+    const int k = ks[c - 1];
+    double best = 1e300;
+    for (int lg = 0; lg <= 4; lg++) {
+      double ms = 0;
+      int rc = lespmv_time_spmm(mh, LESPMV_CSR, 16 + lg, k, 1.0, X.p, k, 0.0, Y.p, k, 1, &ms);
+      if (rc == LESPMV_OK) rc = lespmv_time_spmm(mh, LESPMV_CSR, 16 + lg, k, 1.0, X.p, k, 0.0, Y.p, k, 5, &ms);
+      LSP_REQUIRE(rc == LESPMV_OK, rc, "timing the row engine failed");
+      if (ms < best) {
+        best = ms;
+        sm.lanes_w[c] = 1 << lg;
+      }
+    }
+  }
 }
 }  // namespace
 
@@ -405,6 +436,7 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
     sm.A.reset(reinterpret_cast<Matrix*>(mh));
     sm.h = mh;
     sm.avg_row = (double)nnz / (double)m;
+    const bool tune = lanes == 0;
     if (lanes == 0) {  // the autotuner's lanes per row (rows per wave = 64 / lanes) for this matrix
       int32_t count = 0;
       lespmv_tune_candidates(1 << LESPMV_CSR, S.dtype, nullptr, 0, &count);
@@ -420,6 +452,8 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
         }
     }
     sm.lanes = lanes;
+    for (int c = 0; c < 3; c++) sm.lanes_w[c] = lanes;
+    if (tune) tune_spmm_lanes(S, sm, mh, n);
     sm.bytes = bytes;
     sm.last_use = ++S.tick;
     sm.live = true;
@@ -445,6 +479,15 @@ extern "C" int lespmv_server_matrix(lespmv_server_t h, int32_t id, lespmv_matrix
     LSP_REQUIRE(id >= 0 && id < (int)S.mats.size() && S.mats[id].live, LESPMV_ERR_ARG, "matrix not resident");
     if (A) *A = S.mats[id].h;
     if (lanes) *lanes = S.mats[id].lanes;
+  });
+}
+
+extern "C" int lespmv_server_lanes(lespmv_server_t h, int32_t id, int32_t k, int32_t* lanes) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(id >= 0 && id < (int)S.mats.size() && S.mats[id].live, LESPMV_ERR_ARG, "matrix not resident");
+    LSP_REQUIRE(lanes && k >= 1 && k <= 32, LESPMV_ERR_ARG, "bad arguments");
+    *lanes = Server::lanes_of(S.mats[id], k == 1 ? 1 : k <= 3 ? 2 : 4);
   });
 }
 

@@ -174,6 +174,7 @@ def load() -> C.CDLL:
         "lespmv_server_add_csr": ([vp, i64, i64, i64, vp, vp, vp, i32, C.POINTER(i32)], C.c_int),
         "lespmv_server_remove": ([vp, i32], C.c_int),
         "lespmv_server_matrix": ([vp, i32, C.POINTER(vp), C.POINTER(i32)], C.c_int),
+        "lespmv_server_lanes": ([vp, i32, i32, C.POINTER(i32)], C.c_int),
         "lespmv_server_submit": ([vp, i32, i32, dbl, vp, i64, dbl, vp, i64, C.POINTER(i64)], C.c_int),
         "lespmv_server_poll": ([vp, i32, vp, vp, i32, C.POINTER(i32)], C.c_int),
         "lespmv_server_drain": ([vp], C.c_int),
@@ -590,9 +591,10 @@ class Server:
     def remove(self, mid: int):
         check(load().lespmv_server_remove(self.h, mid))
 
-    def lanes(self, mid: int) -> int:
-        a, lanes = C.c_void_p(), C.c_int32()
-        check(load().lespmv_server_matrix(self.h, mid, C.byref(a), C.byref(lanes)))
+    def lanes(self, mid: int, k: int = 1) -> int:
+        """Lanes per row the server uses for this matrix's requests with k right-hand sides."""
+        lanes = C.c_int32()
+        check(load().lespmv_server_lanes(self.h, mid, k, C.byref(lanes)))
         return lanes.value
 
     def submit(self, mid: int, k: int, alpha: float, X: "DeviceArray", ldx: int, beta: float, Y: "DeviceArray",

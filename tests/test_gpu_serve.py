@@ -55,7 +55,7 @@ def test_packed_kernel_matches_percall_and_oracle(gpu, oracle):
         M.spmm(lib.CSR, k, alpha, dX, k, beta, dY, k)
         assert rel2(Y, dY.numpy()) <= 1e-12, (i, k)
         # the row engine launched on its own (same lanes) gives bitwise the server's result
-        lanes = S.lanes(mids[i])
+        lanes = S.lanes(mids[i], k)
         dY2 = lib.DeviceArray.from_numpy(Y0)
         M.spmm(lib.CSR, k, alpha, dX, k, beta, dY2, k, variant=16 + lanes.bit_length() - 1)
         np.testing.assert_array_equal(Y, dY2.numpy())
@@ -75,11 +75,18 @@ def test_same_result_alone_and_in_mix(gpu):
     mixed = run(lib, S, mids, As, REQS)
     for a, b in zip(alone, mixed):
         np.testing.assert_array_equal(a, b)
-    # a small batch limit splits the mix over several graph launches
+    # a small batch limit splits the mix over several graph launches; lanes
+    # given explicitly apply to every k
     S2 = lib.Server(np.float64, 1 << 30, max_batch=3, slots=2)
     mids2 = [S2.add(A, lanes=S.lanes(m)) for A, m in zip(As, mids)]
-    for a, b in zip(alone, run(lib, S2, mids2, As, REQS)):
+    assert all(S2.lanes(m2, k) == S.lanes(m) for m, m2 in zip(mids, mids2) for k in (1, 2, 16))
+    alone2 = [run(lib, S2, mids2, As, [r])[0] for r in REQS]
+    for a, b in zip(alone2, run(lib, S2, mids2, As, REQS)):
         np.testing.assert_array_equal(a, b)
+    # k = 1 requests use the same lanes on both servers
+    for (i, k, *_), a, b in zip(REQS, alone, alone2):
+        if k == 1:
+            np.testing.assert_array_equal(a, b)
     assert S2.stats()["batches"] >= 3
 
 
