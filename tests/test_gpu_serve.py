@@ -5,7 +5,7 @@ between, and admission / eviction by HBM footprint."""
 import numpy as np
 import pytest
 
-from tests.conftest import rel2
+from tests.conftest import TOL, rel2
 
 pytestmark = pytest.mark.gpu
 
@@ -136,3 +136,41 @@ def test_admission_by_footprint(gpu):
     with pytest.raises(lib.LespmvError) as e:
         S.add(big)
     assert e.value.code == lib.ERR_OOM
+
+
+def test_fp32_server_and_padded_leading_dimensions(gpu, oracle):
+    """FP32 and FP64 servers with requests whose leading dimensions exceed k
+    (scalar X / Y access instead of vectors): each result equals the row engine
+    launched on its own bitwise, leaves the padding columns untouched and is
+    within the dtype's tolerance of the FP64 oracle, column by column."""
+    lib = gpu
+    rng = np.random.default_rng(17)
+    cases = [(0, 3, 5, 4), (1, 8, 11, 9), (0, 17, 17, 20), (1, 32, 33, 32), (0, 1, 2, 3), (1, 4, 4, 4)]
+    for dt in (np.float32, np.float64):
+        As = [lib.generate(lib.GEN_LOGNORMAL, [3000, 2800, 16.0, 1.0, 4096, 0.3, 64], 2, dt),
+              lib.generate(lib.GEN_STENCIL27, [14, 26.0, -1.0], 5, dt)]
+        S = lib.Server(dt, 1 << 30, max_batch=16, slots=2)
+        mids = [S.add(A) for A in As]
+        bufs = []
+        for i, k, ldx, ldy in cases:
+            A = As[i]
+            X = rng.uniform(-1, 1, (A.n, ldx)).astype(dt)
+            Y0 = rng.uniform(-1, 1, (A.m, ldy)).astype(dt)
+            dX, dY = lib.DeviceArray.from_numpy(X), lib.DeviceArray.from_numpy(Y0)
+            S.submit(mids[i], k, 0.9, dX, ldx, 0.5, dY, ldy)
+            bufs.append((X, Y0, dX, dY))
+        S.drain()
+        S.poll()
+        for (i, k, ldx, ldy), (X, Y0, dX, dY) in zip(cases, bufs):
+            A = As[i]
+            Y = dY.numpy()
+            M = lib.Matrix.from_host(A)
+            dY2 = lib.DeviceArray.from_numpy(Y0)
+            M.spmm(lib.CSR, k, 0.9, dX, ldx, 0.5, dY2, ldy, variant=16 + S.lanes(mids[i], k).bit_length() - 1)
+            np.testing.assert_array_equal(Y, dY2.numpy())
+            np.testing.assert_array_equal(Y[:, k:], Y0[:, k:])
+            for j in range(k):
+                ref = Y0[:, j].astype(np.float64)
+                oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v.astype(np.float64), 0.9, X[:, j].astype(np.float64), 0.5,
+                                ref)
+                assert rel2(Y[:, j], ref) <= TOL[np.dtype(dt)], (np.dtype(dt).name, i, k, j)
