@@ -22,7 +22,11 @@ namespace lsp {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMergeItems = 2048;
+constexpr int kMergeItems = 2048;  // merge-path items (row ends + entries) per tile of the CSR kernel
+// items per tile of the column-panel kernel: 1024 for FP64 (555 against
+// 566 us on the lognormal matrix: more workgroups fit), 2048 for FP32
+template <class V>
+constexpr int kPanelItems = sizeof(V) == 8 ? 1024 : 2048;
 
 template <int L, class V>
 __device__ __forceinline__ V group_sum(V v) {
@@ -81,10 +85,10 @@ __device__ int64_t merge_search(int64_t d, const int* rp, int64_t m, int64_t nnz
 }
 
 __global__ void merge_partition_kernel(int64_t tiles, int64_t m, int64_t nnz, const int* __restrict__ rp,
-                                       int* __restrict__ coord) {
+                                       int* __restrict__ coord, int64_t items) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t > tiles) return;
-  int64_t d = t * kMergeItems;
+  int64_t d = t * items;
   if (d > m + nnz) d = m + nnz;
   const int64_t r = merge_search(d, rp, m, nnz);
   coord[2 * t] = (int)r;
@@ -103,7 +107,6 @@ __device__ V block_sum(V v, V* red) {
   return t;
 }
 
-constexpr int kItems = kMergeItems / kBlock;  // nonzeros staged per thread
 constexpr int kLongRow = 48;                     // rows longer than this in a tile are reduced by a wavefront
 
 // Loads of data that is read once: with NT the line is marked for early
@@ -116,16 +119,17 @@ __device__ __forceinline__ T stream_load(const T* p) {
 
 // One merge-path tile b of a CSR matrix (rp, ci, val are the matrix's own
 // arrays, coord its tile starts).
-template <class V, bool NT>
+template <class V, bool NT, int ITEMS>
 __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restrict__ rp,
                                            const int* __restrict__ ci, const V* __restrict__ val,
                                            const V* __restrict__ x, V* __restrict__ y, V alpha, V beta,
                                            const int* __restrict__ coord, int* __restrict__ carry_row,
                                            V* __restrict__ carry_val, V* __restrict__ head_val) {
-  __shared__ V prod[kMergeItems];
-  __shared__ int rows[kMergeItems + 2];
+  constexpr int Q = ITEMS / kBlock;  // entries staged per thread
+  __shared__ V prod[ITEMS];
+  __shared__ int rows[ITEMS + 2];
   __shared__ V red[kBlock / 64];
-  __shared__ int long_rows[kMergeItems / kLongRow + 2];
+  __shared__ int long_rows[ITEMS / kLongRow + 2];
   __shared__ int nlong;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = coord[2 * b], z0 = coord[2 * b + 1];
@@ -134,21 +138,21 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
   const int last = r1 + 1 < m ? r1 + 1 : (int)m;  // rp[r0 .. last] staged
   if (tid == 0) nlong = 0;
   // every load of the tile is issued before the first gather returns
-  int cidx[kItems];
-  V v[kItems];
+  int cidx[Q];
+  V v[Q];
 #pragma unroll
-  for (int q = 0; q < kItems; q++) {
+  for (int q = 0; q < Q; q++) {
     const int i = tid + q * kBlock;
     const bool ok = i < nz;
     cidx[q] = ok ? stream_load<NT>(ci + z0 + i) : 0;
     v[q] = ok ? stream_load<NT>(val + z0 + i) : V(0);
   }
   for (int i = tid; i <= last - r0; i += kBlock) rows[i] = stream_load<NT>(rp + r0 + i);
-  V xv[kItems];
+  V xv[Q];
 #pragma unroll
-  for (int q = 0; q < kItems; q++) xv[q] = (tid + q * kBlock < nz) ? x[cidx[q]] : V(0);
+  for (int q = 0; q < Q; q++) xv[q] = (tid + q * kBlock < nz) ? x[cidx[q]] : V(0);
 #pragma unroll
-  for (int q = 0; q < kItems; q++) {
+  for (int q = 0; q < Q; q++) {
     const int i = tid + q * kBlock;
     if (i < nz) prod[i] = v[q] * xv[q];
   }
@@ -221,8 +225,8 @@ __global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int*
                                                            V beta, const int* __restrict__ coord,
                                                            int* __restrict__ carry_row, V* __restrict__ carry_val,
                                                            V* __restrict__ head_val) {
-  merge_tile<V, false>((int)xcd_block(blockIdx.x, gridDim.x), m, rp, ci, val, x, y, alpha, beta, coord, carry_row,
-                       carry_val, head_val);
+  merge_tile<V, false, kMergeItems>((int)xcd_block(blockIdx.x, gridDim.x), m, rp, ci, val, x, y, alpha, beta, coord,
+                                    carry_row, carry_val, head_val);
 }
 
 // Column panels: the matrix split into P column ranges (P a multiple of the 8
@@ -253,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void panel_merge_kernel(int64_t m, PanelArg
   const int64_t t = u % a.maxT;
   if (p >= a.P || t >= a.tiles[p]) return;
   const int64_t off = a.nzoff[p];
-  merge_tile<V, true>((int)t, m, a.prp + (int64_t)p * (m + 1), ci + off, val + off, x, partial + (int64_t)p * m, V(1),
+  merge_tile<V, true, kPanelItems<V>>((int)t, m, a.prp + (int64_t)p * (m + 1), ci + off, val + off, x, partial + (int64_t)p * m, V(1),
                       V(0), a.coord + (int64_t)p * 2 * (a.maxT + 1), a.carry_row + (int64_t)p * a.maxT,
                       carry_val + (int64_t)p * a.maxT, head_val + (int64_t)p * a.maxT);
 }
@@ -421,7 +425,7 @@ void build_panels(Matrix& A, int P, hipStream_t s) {
   std::vector<int> tiles(P);
   int64_t maxT = 1;
   for (int p = 0; p < P; p++) {
-    tiles[p] = (int)ceil_div(m + nzoff[p + 1] - nzoff[p], kMergeItems);
+    tiles[p] = (int)ceil_div(m + nzoff[p + 1] - nzoff[p], kPanelItems<V>);
     maxT = std::max<int64_t>(maxT, tiles[p]);
   }
   mt.p_maxT = maxT;
@@ -432,7 +436,7 @@ void build_panels(Matrix& A, int P, hipStream_t s) {
     if (tiles[p]) {
       hipLaunchKernelGGL(merge_partition_kernel, grid1d(tiles[p] + 1, kBlock), dim3(kBlock), 0, s, (int64_t)tiles[p],
                          m, nzoff[p + 1] - nzoff[p], mt.p_rp.as<int>() + (int64_t)p * (m + 1),
-                         mt.p_coord.as<int>() + (int64_t)p * 2 * (maxT + 1));
+                         mt.p_coord.as<int>() + (int64_t)p * 2 * (maxT + 1), (int64_t)kPanelItems<V>);
       LSP_LAUNCHED();
     }
   mt.p_carry_row.alloc(sizeof(int) * P * maxT);
@@ -554,7 +558,7 @@ void csr_analyse(Matrix& A, hipStream_t s) {
   mt.carry_val.alloc(vsize(A.dtype) * std::max<int64_t>(mt.merge_tiles, 1));
   mt.head_val.alloc(vsize(A.dtype) * std::max<int64_t>(mt.merge_tiles, 1));
   hipLaunchKernelGGL(merge_partition_kernel, grid1d(mt.merge_tiles + 1, kBlock), dim3(kBlock), 0, s, mt.merge_tiles,
-                     A.m, A.nnz, A.rp.as<int>(), mt.merge_coord.as<int>());
+                     A.m, A.nnz, A.rp.as<int>(), mt.merge_coord.as<int>(), (int64_t)kMergeItems);
   LSP_LAUNCHED();
   mt.analysed = true;
 }
