@@ -47,16 +47,16 @@ inline int rows_item(double avg_row, int lanes, int k, int W, int iters = kItemI
 // Pointers taken from a request descriptor are generic to the compiler; in the
 // global address space it issues global (not flat) loads, which wait on the
 // vector-memory counter alone.
-#define GLOBAL_AS __attribute__((address_space(1)))
+#define LSP_GLOBAL_AS __attribute__((address_space(1)))
 
 // X and Y values of one lane's W right-hand sides: consecutive in memory; one
 // vector access when the request is aligned for it (VEC), else W guarded
 // scalar accesses.
 template <class V, int W, bool VEC>
-__device__ __forceinline__ void load_w(const GLOBAL_AS V* __restrict__ p, int nvalid, V* out) {
+__device__ __forceinline__ void load_w(const LSP_GLOBAL_AS V* __restrict__ p, int nvalid, V* out) {
   if constexpr (VEC && W > 1) {
     typedef V vec_t __attribute__((ext_vector_type(W)));
-    const vec_t v = *(const GLOBAL_AS vec_t*)p;
+    const vec_t v = *(const LSP_GLOBAL_AS vec_t*)p;
 #pragma unroll
     for (int u = 0; u < W; u++) out[u] = v[u];
   } else {
@@ -83,11 +83,11 @@ __device__ __forceinline__ void run_rows(const Q& q, int r0, int r1, int lane) {
   const int P = 64 / G;                       // rows per pass
   const int grp = lane / G, t = lane % G, j = (t % kk) * W, s = t / kk, nzl = G / kk;
   const int nvalid = k - j;
-  const GLOBAL_AS V* __restrict__ val = (const GLOBAL_AS V*)q.val;
-  const GLOBAL_AS V* __restrict__ X = (const GLOBAL_AS V*)q.X + j;
-  GLOBAL_AS V* __restrict__ Y = (GLOBAL_AS V*)q.Y;
-  const GLOBAL_AS int* __restrict__ rp = (const GLOBAL_AS int*)q.rp;
-  const GLOBAL_AS int* __restrict__ ci = (const GLOBAL_AS int*)q.ci;
+  const LSP_GLOBAL_AS V* __restrict__ val = (const LSP_GLOBAL_AS V*)q.val;
+  const LSP_GLOBAL_AS V* __restrict__ X = (const LSP_GLOBAL_AS V*)q.X + j;
+  LSP_GLOBAL_AS V* __restrict__ Y = (LSP_GLOBAL_AS V*)q.Y;
+  const LSP_GLOBAL_AS int* __restrict__ rp = (const LSP_GLOBAL_AS int*)q.rp;
+  const LSP_GLOBAL_AS int* __restrict__ ci = (const LSP_GLOBAL_AS int*)q.ci;
   const int64_t ldx = q.ldx;
   const double alpha = q.alpha, beta = q.beta;
   for (int base = r0; base < r1; base += P) {
@@ -130,7 +130,7 @@ __device__ __forceinline__ void run_rows(const Q& q, int r0, int r1, int lane) {
       sum[u] = a;
     }
     if (ok && s == 0) {
-      GLOBAL_AS V* y = Y + (int64_t)row * q.ldy + j;
+      LSP_GLOBAL_AS V* y = Y + (int64_t)row * q.ldy + j;
       if constexpr (VEC && W > 1) {
         typedef V vec_t __attribute__((ext_vector_type(W)));
         vec_t o;
@@ -138,11 +138,11 @@ __device__ __forceinline__ void run_rows(const Q& q, int r0, int r1, int lane) {
 #pragma unroll
           for (int u = 0; u < W; u++) o[u] = (V)(alpha * sum[u]);
         } else {
-          const vec_t old = *(const GLOBAL_AS vec_t*)y;
+          const vec_t old = *(const LSP_GLOBAL_AS vec_t*)y;
 #pragma unroll
           for (int u = 0; u < W; u++) o[u] = (V)(alpha * sum[u] + beta * (double)old[u]);
         }
-        *(GLOBAL_AS vec_t*)y = o;
+        *(LSP_GLOBAL_AS vec_t*)y = o;
       } else {
 #pragma unroll
         for (int u = 0; u < W; u++)
