@@ -86,10 +86,15 @@ def measure(lib, M, dx, dy, section, wanted, warmup):
             log(f"[{section}] {label}: skipped ({e})")
             continue
         eff = M.effective_bytes(fmt, 1, False)
+        try:  # the reference's own formula (include/spmv_benchmark.h:26-160), for comparison only
+            ref_bytes = M.reference_bytes(fmt)
+        except lib.LespmvError:
+            ref_bytes = None
         M.time_spmv(fmt, 1.0, dx, 0.0, dy, max(warmup, 1), variant)
         ms = M.time_spmv(fmt, 1.0, dx, 0.0, dy, 50, variant)
         results.append(dict(label=label, fmt=fmt, variant=variant, params=params, ms=ms, gbs=eff / ms / 1e6,
-                            effective_bytes=eff, convert_ms=conv_ms, format_bytes=M.format_bytes(fmt)))
+                            effective_bytes=eff, convert_ms=conv_ms, format_bytes=M.format_bytes(fmt),
+                            ref_gbs=ref_bytes / ms / 1e6 if ref_bytes else None))
         log(f"[{section}] {label}: {ms * 1e3:.1f} us  {eff / ms / 1e6:.0f} GB/s  convert {conv_ms:.1f} ms")
         if fmt != lib.CSR:
             M.release(fmt)
@@ -99,7 +104,8 @@ def measure(lib, M, dx, dy, section, wanted, warmup):
 def summary(results):
     return {r["label"]: {"us": round(r["ms"] * 1e3, 2), "GBs": round(r["gbs"], 1), "frac": round(r["gbs"] * 1e9 /
                                                                                                   HBM_PEAK, 4),
-                         "convert_ms": round(r["convert_ms"], 2), "format_bytes": r["format_bytes"]}
+                         "convert_ms": round(r["convert_ms"], 2), "format_bytes": r["format_bytes"],
+                         "GBs_reference_formula": round(r["ref_gbs"], 1) if r["ref_gbs"] else None}
             for r in results}
 
 
