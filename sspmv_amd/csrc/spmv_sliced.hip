@@ -99,19 +99,38 @@ __global__ __launch_bounds__(kBlock) void sliced_spmv_kernel(int64_t m, int64_t 
   const int* cp = col + coff[un.x] + r;
   const V* vp = sv + coff[un.x] + r;
   V sum = 0;
+  // kUnroll slots per step; the next step's column and value loads are
+  // issued before this step's gathers are used, so they overlap
+  int c[kUnroll];
+  V v[kUnroll];
+#pragma unroll
+  for (int q = 0; q < kUnroll; q++) {
+    const bool ok = un.y + q < un.z;
+    c[q] = ok ? cp[(int64_t)(un.y + q) * C] : -1;
+    v[q] = ok ? vp[(int64_t)(un.y + q) * C] : V(0);
+  }
   for (int j = un.y; j < un.z; j += kUnroll) {
-    int c[kUnroll];
-    V v[kUnroll];
+    V xv[kUnroll];
+#pragma unroll
+    for (int q = 0; q < kUnroll; q++) xv[q] = c[q] >= 0 ? x[c[q]] : V(0);
+    const bool more = c[kUnroll - 1] >= 0 && j + kUnroll < un.z;
+    int cn[kUnroll];
+    V vn[kUnroll];
 #pragma unroll
     for (int q = 0; q < kUnroll; q++) {
-      const bool ok = j + q < un.z;
-      c[q] = ok ? cp[(int64_t)(j + q) * C] : -1;
-      v[q] = ok ? vp[(int64_t)(j + q) * C] : V(0);
+      const bool ok = more && j + kUnroll + q < un.z;
+      cn[q] = ok ? cp[(int64_t)(j + kUnroll + q) * C] : -1;
+      vn[q] = ok ? vp[(int64_t)(j + kUnroll + q) * C] : V(0);
     }
 #pragma unroll
     for (int q = 0; q < kUnroll; q++)
-      if (c[q] >= 0) sum += v[q] * x[c[q]];
-    if (c[kUnroll - 1] < 0) break;
+      if (c[q] >= 0) sum += v[q] * xv[q];
+    if (!more) break;
+#pragma unroll
+    for (int q = 0; q < kUnroll; q++) {
+      c[q] = cn[q];
+      v[q] = vn[q];
+    }
   }
   if (un.w < 0) {
     const int64_t out = order ? order[row] : row;
