@@ -74,7 +74,7 @@ def test_csr_edge_shapes(gpu, oracle):
         x = rng.uniform(-1, 1, A.n)
         y0 = rng.uniform(-1, 1, A.m)
         ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
-        for var in (1, 2, 4, 5, 22):
+        for var in (1, 2, 4, 5, 17, 19, 21, 22):
             y = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
             assert rel2(y, ref) <= 1e-12, (name, var)
 
