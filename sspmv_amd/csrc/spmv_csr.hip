@@ -290,6 +290,21 @@ __global__ void panel_reduce(int64_t m, int P, const V* __restrict__ partial, V*
   y[r] = axpby(alpha, s, beta, y + r);
 }
 
+// The same sum with 16-byte loads of each partial: for M a multiple of
+// 16 / sizeof(V), so that every panel's partial starts 16-byte aligned.
+template <class V>
+__global__ void panel_reduce_vec(int64_t m, int P, const V* __restrict__ partial, V* __restrict__ y, V alpha,
+                                 V beta) {
+  constexpr int W = 16 / sizeof(V);
+  typedef V vec_t __attribute__((ext_vector_type(W)));
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g * W >= m) return;
+  vec_t s = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(partial) + g);
+  for (int p = 1; p < P; p++) s += __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(partial + (int64_t)p * m) + g);
+#pragma unroll
+  for (int u = 0; u < W; u++) y[g * W + u] = axpby(alpha, s[u], beta, y + g * W + u);
+}
+
 // per (panel, row) entry counts, each row's counts by its own thread
 __global__ void panel_count(int64_t m, int P, int64_t width, const int* __restrict__ rp, const int* __restrict__ ci,
                             int64_t* __restrict__ cnt) {
@@ -460,8 +475,13 @@ void panel_run(Matrix& A, int P, double alpha, const void* x, double beta, void*
   hipLaunchKernelGGL((panel_fixup_kernel<V>), grid1d(P * mt.p_maxT, kBlock), dim3(kBlock), 0, s, A.m, a,
                      mt.p_carry_val.as<V>(), mt.p_head.as<V>(), mt.p_partial.as<V>());
   LSP_LAUNCHED();
-  hipLaunchKernelGGL((panel_reduce<V>), grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, P, mt.p_partial.as<V>(),
-                     static_cast<V*>(y), (V)alpha, (V)beta);
+  constexpr int W = 16 / sizeof(V);
+  if (A.m % W == 0)
+    hipLaunchKernelGGL((panel_reduce_vec<V>), grid1d(A.m / W, kBlock), dim3(kBlock), 0, s, A.m, P,
+                       mt.p_partial.as<V>(), static_cast<V*>(y), (V)alpha, (V)beta);
+  else
+    hipLaunchKernelGGL((panel_reduce<V>), grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, P, mt.p_partial.as<V>(),
+                       static_cast<V*>(y), (V)alpha, (V)beta);
   LSP_LAUNCHED();
 }
 
