@@ -27,11 +27,11 @@ def lib_params(gpu, fmt, prm):
 
 def gpu_native_params(fmt):
     if fmt == SELL:
-        return [[64, 0, 1, 0]]
+        return [[64, 0, 1, 0], [32, 0, 1, 0]]
     if fmt == SELL_C_SIGMA:
-        return [[64, 4096, 1, 0], [64, 128, 1, 0]]
+        return [[64, 4096, 1, 0], [64, 128, 1, 0], [32, 256, 1, 0]]
     if fmt == SELL_C_R:
-        return [[64, 0, 1, 0]]
+        return [[64, 0, 1, 0], [32, 0, 1, 0]]
     if fmt == ELL:
         return [[1, 0, 0, 0]]
     if fmt == BSR:

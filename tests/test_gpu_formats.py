@@ -127,6 +127,35 @@ def test_sell_sigma_must_divide(gpu):
     assert e.value.code == gpu.ERR_UNSUPPORTED
 
 
+@pytest.mark.parametrize("chunk", [64, 32, 4])
+def test_sell_pads_never_multiply_x(gpu, chunk):
+    """Pads (col -1, value 0) add nothing even where x holds inf: only row 0
+    uses column 0 and x[0] = inf, so every other row stays finite and exact."""
+    rng = np.random.default_rng(chunk)
+    m = n = 300
+    rows = [[0, 1]] + [sorted(set([i] + list(rng.integers(1, n, i % 7)))) for i in range(1, m)]
+    rp = np.zeros(m + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate([np.array(r, np.int32) for r in rows])
+    v = rng.uniform(0.5, 1.5, len(ci))
+    A = HostCsr(m, n, rp, ci, v)
+    x = rng.uniform(-1, 1, n)
+    x[0] = np.inf
+    ref = np.array([sum(v[j] * x[ci[j]] for j in range(rp[i], rp[i + 1])) for i in range(m)])
+    M = gpu.Matrix.from_host(A)
+    for fmt, prm in ((SELL_C_SIGMA, gpu.make_params(chunk=chunk, sigma=4 * chunk, alignment=1)),
+                     (SELL_C_R, gpu.make_params(chunk=chunk, alignment=1))):
+        M.convert(fmt, prm)
+        dx = gpu.DeviceArray.from_numpy(x)
+        dy = gpu.DeviceArray.from_numpy(np.zeros(m))
+        M.spmv(fmt, 1.0, dx, 0.0, dy)
+        y = dy.numpy()
+        assert np.isposinf(y[0]), (fmt, chunk, y[0])
+        assert np.all(np.isfinite(y[1:])), (fmt, chunk)
+        assert rel2(y[1:], ref[1:]) <= 1e-12, (fmt, chunk)
+        M.release(fmt)
+
+
 def test_not_converted_is_an_error(gpu):
     A = suite()["rand200"]
     M = gpu.Matrix.from_host(A)
