@@ -22,9 +22,12 @@ namespace lsp {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMergeItems = 2048;  // merge-path items (row ends + entries) per tile of the CSR kernel
+// merge-path items (row ends + entries) per tile of the CSR kernel: 1024
+// (stencil 369 against 374 us with 2048, FP32 lognormal 692 against 699)
+constexpr int kMergeItems = 1024;
 // items per tile of the column-panel kernel: 1024 for FP64 (555 against
-// 566 us on the lognormal matrix: more workgroups fit), 2048 for FP32
+// 566 us on the lognormal matrix: more workgroups fit), 2048 for FP32 (431
+// against 449 us with 1024)
 template <class V>
 constexpr int kPanelItems = sizeof(V) == 8 ? 1024 : 2048;
 
