@@ -9,7 +9,7 @@ from tests.conftest import TOL, rel2
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MODEL = os.path.join(ROOT, "sspmv_amd", "selector_data", "selector_mi355x.bin")
-DB = os.path.join(ROOT, "sspmv_amd", "selector_data", "timings_mi355x.npz")
+HELDOUT = os.path.join(ROOT, "sspmv_amd", "selector_data", "heldout_mi355x.npz")
 
 
 def test_tune_grid_and_autotune(gpu, oracle):
@@ -91,17 +91,19 @@ def test_selector_small_pipeline(gpu, tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained selector committed")
 def test_trained_selector_beats_default(gpu):
-    """The committed selector on the held-out 20 % of the committed timing
-    database: it must gain over the library default (CSR with the automatic
-    kernel), i.e. more matrices within 10 % of the exhaustive best and a lower
-    mean regret."""
+    """The committed selector on the committed held-out database (1,500
+    matrices collected apart from its training data): the pick is within 10 %
+    of the exhaustive best on at least 90 % of them, and it gains over the
+    library default (CSR with the automatic kernel) in that share and in mean
+    regret."""
     from sspmv_amd import selector
     lib = gpu
-    db = selector.load_db(DB)
-    _, te = selector.split(len(db["X"]))
+    db = selector.load_db(HELDOUT)
     mlp = lib.Mlp(path=MODEL)
-    cand = selector.picks(mlp, db["X"][te].astype(np.float32), db["allowed"][te])
-    sel = selector.evaluate(db["times"][te], cand)
-    base = selector.evaluate(db["times"][te], np.full(len(te), selector.N_GRID))
+    cand = selector.picks(mlp, db["X"].astype(np.float32), db["allowed"])
+    sel = selector.evaluate(db["times"], cand)
+    base = selector.evaluate(db["times"], np.full(len(db["X"]), selector.N_GRID))
+    print(f"held-out within 10 %: selector {sel['within_10pct']:.3f}, default {base['within_10pct']:.3f}")
+    assert sel["within_10pct"] >= 0.90
     assert sel["within_10pct"] > base["within_10pct"]
     assert sel["mean_regret"] < base["mean_regret"]
