@@ -7,7 +7,7 @@
 //  * vector: L lanes of a 64-wide wavefront per row (L = 2..64 tuned from the
 //    row-length distribution), unit-stride loads of col/val, DPP/permute
 //    reduction inside the L-lane group;
-//  * merge: merge-path tiles of 2048 (row end + nonzero) items, so every
+//  * merge: merge-path tiles of 1024 (row end + nonzero) items, so every
 //    workgroup gets the same amount of work whatever the row lengths. The tile's
 //    products are staged in LDS, complete rows are reduced from LDS, and the row
 //    open at a tile edge is carried to a small deterministic fix-up kernel.
