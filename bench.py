@@ -56,9 +56,10 @@ def kernels(lib, section):
                 ("BSR-4x4", lib.BSR, 0, lib.make_params(block_rows=4, block_cols=4))]
     ks = [("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None), ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None),
           ("CSR-panel8", lib.CSR, lib.VARIANT_PANEL, None), ("CSR-panel16", lib.CSR, lib.VARIANT_PANEL16, None),
+          ("CSR-panel4", lib.CSR, lib.VARIANT_PANEL4, None),
           ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, sell(64, 4096)), ("SELL-64-R", lib.SELL_C_R, 0, sell(64, 0)),
           ("COO", lib.COO, 0, None), ("CSR5-64x16", lib.CSR5, 0, lib.make_params(omega=64, sigma=16))]
-    return ks if section == "lognormal" else ks[:5]
+    return ks if section == "lognormal" else ks[:6]
 
 
 def traffic_from_profiles(label):

@@ -16,7 +16,8 @@
 //  * LeSpMV_<fmt> with host x/y stages them through device buffers; the
 //    *_device overloads take device pointers and do no copies;
 //  * kernel_flag / kernel_tag select the GPU kernel variant
-//    (0 auto, 1 vector, 2 merge / LDS window, 3 COO atomics);
+//    (0 auto, 1 vector, 2 merge / LDS window, 3 COO atomics, 4 / 5 / 6 CSR
+//    column panels: 8 / 16 / 4);
 //  * errors throw lespmv::Error instead of calling exit();
 //  * IndexType may be int or long long; long long matrices must fit 32-bit
 //    indices on the device.

@@ -76,7 +76,10 @@ enum {
   LESPMV_VARIANT_ATOMIC = 3,    /* COO: atomics instead of the carry fix-up pass */
   LESPMV_VARIANT_PANEL = 4,     /* CSR: 8 column panels, one per XCD (x slice held in that XCD's L2),
                                    merge-path per panel, partial y reduced in panel order */
-  LESPMV_VARIANT_PANEL16 = 5    /* CSR: the same with 16 panels (two per XCD) */
+  LESPMV_VARIANT_PANEL16 = 5,   /* CSR: the same with 16 panels (two per XCD) */
+  LESPMV_VARIANT_PANEL4 = 6     /* CSR: 4 panels, each run by two XCDs; the 4 MB L2 of an XCD holds an
+                                   x slice when x is about 16 MB (FP32 x of 4M columns), as 8 panels
+                                   do for 32 MB (FP64) */
 };
 
 /* Conversion parameters. Fields a format does not use are ignored. */

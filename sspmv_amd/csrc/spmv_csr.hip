@@ -242,6 +242,7 @@ __global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int*
 // panel order.
 struct PanelArgs {
   int P;
+  int share;  // XCDs per panel when P < 8 (P divides 8), else 1
   int64_t maxT;
   const int64_t* nzoff;  // [P + 1] first entry of each panel
   const int* prp;        // [P][m + 1] panel-local row pointers
@@ -256,8 +257,11 @@ __global__ __launch_bounds__(kBlock) void panel_merge_kernel(int64_t m, PanelArg
                                                              V* __restrict__ partial, V* __restrict__ carry_val,
                                                              V* __restrict__ head_val) {
   const int64_t u = blockIdx.x / kXcds;
-  const int p = (int)(blockIdx.x % kXcds + kXcds * (u / a.maxT));
-  const int64_t t = u % a.maxT;
+  const int xcd = (int)(blockIdx.x % kXcds);
+  // P >= 8: XCD b % 8 runs panels b % 8, b % 8 + 8, ...; P < 8: `share`
+  // XCDs run one panel, taking its tiles in turn
+  const int p = a.share > 1 ? xcd / a.share : (int)(xcd + kXcds * (u / a.maxT));
+  const int64_t t = a.share > 1 ? u * a.share + xcd % a.share : u % a.maxT;
   if (p >= a.P || t >= a.tiles[p]) return;
   const int64_t off = a.nzoff[p];
   merge_tile<V, true, kPanelItems<V>>((int)t, m, a.prp + (int64_t)p * (m + 1), ci + off, val + off, x, partial + (int64_t)p * m, V(1),
@@ -469,9 +473,11 @@ template <class V>
 void panel_run(Matrix& A, int P, double alpha, const void* x, double beta, void* y, hipStream_t s) {
   CsrMeta& mt = A.meta;
   if (mt.panels != P) build_panels<V>(A, P, s);
-  PanelArgs a{P, mt.p_maxT, mt.p_nzoff.as<int64_t>(), mt.p_rp.as<int>(), mt.p_tiles.as<int>(), mt.p_coord.as<int>(),
-              mt.p_carry_row.as<int>()};
-  hipLaunchKernelGGL((panel_merge_kernel<V>), dim3((unsigned)(P * mt.p_maxT)), dim3(kBlock), 0, s, A.m, a,
+  const int share = P < kXcds ? kXcds / P : 1;
+  PanelArgs a{P, share, mt.p_maxT, mt.p_nzoff.as<int64_t>(), mt.p_rp.as<int>(), mt.p_tiles.as<int>(),
+              mt.p_coord.as<int>(), mt.p_carry_row.as<int>()};
+  const int64_t blocks = share > 1 ? kXcds * ceil_div(mt.p_maxT, share) : P * mt.p_maxT;
+  hipLaunchKernelGGL((panel_merge_kernel<V>), dim3((unsigned)blocks), dim3(kBlock), 0, s, A.m, a,
                      mt.p_ci.as<int>(), mt.p_val.as<V>(), static_cast<const V*>(x), mt.p_partial.as<V>(),
                      mt.p_carry_val.as<V>(), mt.p_head.as<V>());
   LSP_LAUNCHED();
@@ -595,8 +601,8 @@ void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, 
     else merge_run<float>(A, alpha, x, beta, y, s);
     return;
   }
-  if (variant == LESPMV_VARIANT_PANEL || variant == LESPMV_VARIANT_PANEL16) {
-    const int P = variant == LESPMV_VARIANT_PANEL ? 8 : 16;
+  if (variant == LESPMV_VARIANT_PANEL || variant == LESPMV_VARIANT_PANEL16 || variant == LESPMV_VARIANT_PANEL4) {
+    const int P = variant == LESPMV_VARIANT_PANEL ? 8 : variant == LESPMV_VARIANT_PANEL16 ? 16 : 4;
     if (A.dtype == LESPMV_F64) panel_run<double>(A, P, alpha, x, beta, y, s);
     else panel_run<float>(A, P, alpha, x, beta, y, s);
     return;

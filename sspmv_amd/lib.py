@@ -20,6 +20,7 @@ COO, CSR, DIA, ELL, SELL, SELL_C_SIGMA, SELL_C_R, CSR5, BSR = range(9)
 FORMAT_NAMES = ["COO", "CSR", "DIA", "ELL", "S-ELL", "SELL-c-sigma", "SELL-c-R", "CSR5", "BSR"]
 F32, F64 = 0, 1
 VARIANT_AUTO, VARIANT_VECTOR, VARIANT_MERGE, VARIANT_ATOMIC, VARIANT_PANEL, VARIANT_PANEL16 = 0, 1, 2, 3, 4, 5
+VARIANT_PANEL4 = 6
 GEN_LAPLACE2D, GEN_STENCIL27, GEN_LOGNORMAL, GEN_BLOCKRAND, GEN_BANDED, GEN_POWERLAW, GEN_BLOCKDIAG, GEN_UNIFORM = \
     range(8)
 

@@ -12,7 +12,7 @@ from tests.matrices import HostCsr, random_csr, suite
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [0, 1, 2, 4, 5, 17, 18, 19, 20, 21, 22]  # auto, vector(tuned), merge, panels 8 / 16, vector L=2..64
+VARIANTS = [0, 1, 2, 4, 5, 6, 17, 18, 19, 20, 21, 22]  # auto, vector(tuned), merge, panels 8 / 16 / 4, vector L=2..64
 
 
 def oracle_y(oracle, A, alpha, x, beta, y0):
@@ -74,7 +74,7 @@ def test_csr_edge_shapes(gpu, oracle):
         x = rng.uniform(-1, 1, A.n)
         y0 = rng.uniform(-1, 1, A.m)
         ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
-        for var in (1, 2, 4, 5, 17, 19, 21, 22):
+        for var in (1, 2, 4, 5, 6, 17, 19, 21, 22):
             y = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
             assert rel2(y, ref) <= 1e-12, (name, var)
 
@@ -89,7 +89,7 @@ def test_csr_generated_configs(gpu, oracle, dt):
         x = gpu.fill_uniform(A.n, 4, dt)
         y0 = gpu.fill_uniform(A.m, 9, dt)
         ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
-        for var in (0, 1, 2, 4, 5):
+        for var in (0, 1, 2, 4, 5, 6):
             y = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
             err = rel2(y, ref)
             print(f"{np.dtype(dt).name} fam={fam} variant={var} rel2={err:.3e}")
