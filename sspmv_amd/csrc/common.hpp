@@ -170,7 +170,7 @@ struct CsrMeta {
   // column panels (CSR panel variants): P column ranges, each its own CSR
   int32_t panels = 0;
   int64_t p_maxT = 0;
-  DBuf p_nzoff, p_rp, p_ci, p_val, p_tiles, p_coord, p_carry_row, p_carry_val, p_head, p_partial;
+  DBuf p_nzoff, p_rp, p_ci, p_val, p_tiles, p_coord, p_carry_row, p_carry_val, p_head, p_partial, p_ends, p_endoff;
 };
 
 struct Matrix {

@@ -121,13 +121,17 @@ __device__ __forceinline__ T stream_load(const T* p) {
 }
 
 // One merge-path tile b of a CSR matrix (rp, ci, val are the matrix's own
-// arrays, coord its tile starts).
-template <class V, bool NT, int ITEMS>
+// arrays, coord its tile starts). With BITS the tile's row boundaries come
+// from `ends`, one bit per merge-path item set at row ends (row r ends at item
+// r + rp[r + 1]), instead of from the row pointers: ITEMS / 32 words per tile
+// against four bytes per row.
+template <class V, bool NT, int ITEMS, bool BITS = false>
 __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restrict__ rp,
                                            const int* __restrict__ ci, const V* __restrict__ val,
                                            const V* __restrict__ x, V* __restrict__ y, V alpha, V beta,
                                            const int* __restrict__ coord, int* __restrict__ carry_row,
-                                           V* __restrict__ carry_val, V* __restrict__ head_val) {
+                                           V* __restrict__ carry_val, V* __restrict__ head_val,
+                                           const uint32_t* __restrict__ ends = nullptr) {
   constexpr int Q = ITEMS / kBlock;  // entries staged per thread
   __shared__ V prod[ITEMS];
   __shared__ int rows[ITEMS + 2];
@@ -150,7 +154,32 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
     cidx[q] = ok ? stream_load<NT>(ci + z0 + i) : 0;
     v[q] = ok ? stream_load<NT>(val + z0 + i) : V(0);
   }
-  for (int i = tid; i <= last - r0; i += kBlock) rows[i] = stream_load<NT>(rp + r0 + i);
+  if constexpr (BITS) {
+    // wave 0: the tile's items start at r0 + z0 = b * ITEMS (a whole word);
+    // the j-th row end of the tile, at item k of it, closes row r0 + j, whose
+    // successor starts at entry z0 + k - j
+    static_assert(ITEMS % 32 == 0 && ITEMS / 32 <= 64, "one wave holds the tile's words");
+    if (wave == 0) {
+      const int64_t w0 = ((int64_t)r0 + z0) >> 5;
+      uint32_t w = lane < ITEMS / 32 ? stream_load<NT>(ends + w0 + lane) : 0u;
+      const int cnt = __popc(w);
+      int pre = cnt;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(pre, off);
+        if (lane >= off) pre += t;
+      }
+      int j = pre - cnt;
+      while (w) {
+        const int q = __ffs(w) - 1;
+        w &= w - 1;
+        rows[j + 1] = z0 + lane * 32 + q - j;
+        ++j;
+      }
+      if (lane == 0) rows[0] = stream_load<NT>(rp + r0);
+    }
+  } else {
+    for (int i = tid; i <= last - r0; i += kBlock) rows[i] = stream_load<NT>(rp + r0 + i);
+  }
   V xv[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) xv[q] = (tid + q * kBlock < nz) ? x[cidx[q]] : V(0);
@@ -249,6 +278,8 @@ struct PanelArgs {
   const int* tiles;      // [P] merge tiles per panel
   const int* coord;      // [P][2 (maxT + 1)]
   int* carry_row;        // [P][maxT]
+  const uint32_t* ends;  // row-end bits of every panel's merge-path items
+  const int64_t* endoff; // [P] first word of each panel's bits
 };
 
 template <class V>
@@ -264,9 +295,11 @@ __global__ __launch_bounds__(kBlock) void panel_merge_kernel(int64_t m, PanelArg
   const int64_t t = a.share > 1 ? u * a.share + xcd % a.share : u % a.maxT;
   if (p >= a.P || t >= a.tiles[p]) return;
   const int64_t off = a.nzoff[p];
-  merge_tile<V, true, kPanelItems<V>>((int)t, m, a.prp + (int64_t)p * (m + 1), ci + off, val + off, x, partial + (int64_t)p * m, V(1),
-                      V(0), a.coord + (int64_t)p * 2 * (a.maxT + 1), a.carry_row + (int64_t)p * a.maxT,
-                      carry_val + (int64_t)p * a.maxT, head_val + (int64_t)p * a.maxT);
+  merge_tile<V, true, kPanelItems<V>, true>((int)t, m, a.prp + (int64_t)p * (m + 1), ci + off, val + off, x,
+                                             partial + (int64_t)p * m, V(1), V(0),
+                                             a.coord + (int64_t)p * 2 * (a.maxT + 1),
+                                             a.carry_row + (int64_t)p * a.maxT, carry_val + (int64_t)p * a.maxT,
+                                             head_val + (int64_t)p * a.maxT, a.ends + a.endoff[p]);
 }
 
 template <class V>
@@ -310,6 +343,18 @@ __global__ void panel_reduce_vec(int64_t m, int P, const V* __restrict__ partial
   for (int p = 1; p < P; p++) s += __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(partial + (int64_t)p * m) + g);
 #pragma unroll
   for (int u = 0; u < W; u++) y[g * W + u] = axpby(alpha, s[u], beta, y + g * W + u);
+}
+
+// row-end bits of the panels' merge paths: row i of panel p ends at item
+// i + rp_p[i + 1]
+__global__ void panel_ends(int64_t m, int P, const int* __restrict__ prp, const int64_t* __restrict__ endoff,
+                           uint32_t* __restrict__ ends) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)P * m) return;
+  const int p = (int)(g / m);
+  const int64_t i = g - (int64_t)p * m;
+  const int64_t k = i + prp[(int64_t)p * (m + 1) + i + 1];
+  atomicOr(ends + endoff[p] + (k >> 5), 1u << (k & 31));
 }
 
 // per (panel, row) entry counts, each row's counts by its own thread
@@ -461,6 +506,19 @@ void build_panels(Matrix& A, int P, hipStream_t s) {
                          mt.p_coord.as<int>() + (int64_t)p * 2 * (maxT + 1), (int64_t)kPanelItems<V>);
       LSP_LAUNCHED();
     }
+  // row-end bits: tiles[p] * ITEMS / 32 words per panel, so every tile reads
+  // whole words inside its panel's run
+  std::vector<int64_t> endoff(P + 1, 0);
+  for (int p = 0; p < P; p++) endoff[p + 1] = endoff[p] + (int64_t)tiles[p] * (kPanelItems<V> / 32);
+  mt.p_ends.alloc(sizeof(uint32_t) * std::max<int64_t>(endoff[P], 1));
+  LSP_HIP(hipMemsetAsync(mt.p_ends.p, 0, mt.p_ends.bytes, s));
+  mt.p_endoff.alloc(sizeof(int64_t) * (P + 1));
+  LSP_HIP(hipMemcpyAsync(mt.p_endoff.p, endoff.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
+  if (m) {
+    hipLaunchKernelGGL(panel_ends, grid1d((int64_t)P * m, kBlock), dim3(kBlock), 0, s, m, P, mt.p_rp.as<int>(),
+                       mt.p_endoff.as<int64_t>(), mt.p_ends.as<uint32_t>());
+    LSP_LAUNCHED();
+  }
   mt.p_carry_row.alloc(sizeof(int) * P * maxT);
   mt.p_carry_val.alloc(sizeof(V) * P * maxT);
   mt.p_head.alloc(sizeof(V) * P * maxT);
@@ -474,8 +532,16 @@ void panel_run(Matrix& A, int P, double alpha, const void* x, double beta, void*
   CsrMeta& mt = A.meta;
   if (mt.panels != P) build_panels<V>(A, P, s);
   const int share = P < kXcds ? kXcds / P : 1;
-  PanelArgs a{P, share, mt.p_maxT, mt.p_nzoff.as<int64_t>(), mt.p_rp.as<int>(), mt.p_tiles.as<int>(),
-              mt.p_coord.as<int>(), mt.p_carry_row.as<int>()};
+  PanelArgs a{P,
+              share,
+              mt.p_maxT,
+              mt.p_nzoff.as<int64_t>(),
+              mt.p_rp.as<int>(),
+              mt.p_tiles.as<int>(),
+              mt.p_coord.as<int>(),
+              mt.p_carry_row.as<int>(),
+              mt.p_ends.as<uint32_t>(),
+              mt.p_endoff.as<int64_t>()};
   const int64_t blocks = share > 1 ? kXcds * ceil_div(mt.p_maxT, share) : P * mt.p_maxT;
   hipLaunchKernelGGL((panel_merge_kernel<V>), dim3((unsigned)blocks), dim3(kBlock), 0, s, A.m, a,
                      mt.p_ci.as<int>(), mt.p_val.as<V>(), static_cast<const V*>(x), mt.p_partial.as<V>(),
