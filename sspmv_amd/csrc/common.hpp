@@ -171,6 +171,9 @@ struct CsrMeta {
   int32_t panels = 0;
   int64_t p_maxT = 0;
   DBuf p_nzoff, p_rp, p_ci, p_val, p_tiles, p_coord, p_carry_row, p_carry_val, p_head, p_partial, p_ends, p_endoff;
+  int64_t p_width = 0;       // columns per panel
+  bool p_compact = false;    // FP64 columns kept as 16 bits + a nibble (p_width <= 2^20), p_ci freed
+  DBuf p_ci16, p_cihi;
 };
 
 struct Matrix {

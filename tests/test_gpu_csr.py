@@ -170,3 +170,27 @@ def test_csr_merge_wide_rows(gpu, oracle, dt):
         ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
         got = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, gpu.VARIANT_MERGE)
         assert rel2(got, ref) <= TOL[np.dtype(dt)], (A.m, A.nnz)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_csr_panels_wide_and_compact_columns(gpu, oracle, dt):
+    """Column panels on a matrix of 9M columns: in FP64, 16 panels (562,500
+    columns each) keep panel-local columns in 16 bits plus a nibble, and 8
+    and 4 panels (1,125,000 and 2,250,000 columns, beyond 2^20) keep them in
+    32 bits; FP32 panels always keep 32 bits. All against the oracle."""
+    rng = np.random.default_rng(21)
+    m, n = 3000, 9_000_000
+    lens = rng.poisson(12, m)
+    lens[::50] = 0
+    rows = [np.unique(rng.integers(0, n, int(k))) for k in lens]
+    rp = np.zeros(m + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    A = HostCsr(m, n, rp, ci, rng.uniform(-1, 1, len(ci)).astype(dt))
+    M = gpu.Matrix.from_host(A)
+    x = rng.uniform(-1, 1, n).astype(dt)
+    y0 = rng.uniform(-1, 1, m).astype(dt)
+    ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+    for var in (gpu.VARIANT_PANEL16, gpu.VARIANT_PANEL, gpu.VARIANT_PANEL4):
+        got = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
+        assert rel2(got, ref) <= TOL[np.dtype(dt)], (np.dtype(dt).name, var, rel2(got, ref))
