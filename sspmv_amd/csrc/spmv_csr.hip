@@ -175,9 +175,9 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
       uint32_t w = lane < ITEMS / 32 ? stream_load<NT>(ends + w0 + lane) : 0u;
       const int cnt = __popc(w);
       int pre = cnt;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int t = __shfl_up(pre, off);
-        if (lane >= off) pre += t;
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(pre, d);
+        if (lane >= d) pre += t;
       }
       int j = pre - cnt;
       while (w) {
