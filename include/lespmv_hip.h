@@ -27,7 +27,12 @@
  * gives the message of the last failure on the calling thread. Nothing exits.
  * Threading: a matrix handle may be used from one thread at a time; different
  * handles are independent. Every compute call takes a stream (NULL = the
- * handle's own stream) and is asynchronous unless stated otherwise.
+ * handle's own stream) and is asynchronous unless stated otherwise. A handle's
+ * device scratch (merge-path and panel partials, fix-up carries) serves one
+ * call at a time: a call on another stream than the handle's previous call is
+ * ordered after that call (an event recorded on the caller's stream after each
+ * call there, waited for by the next stream). A caller's stream must stay alive
+ * until the handle's next call has been issued.
  */
 #ifndef LESPMV_HIP_H
 #define LESPMV_HIP_H
@@ -70,7 +75,9 @@ enum {
 
 /* Kernel variants (the GPU meaning of the reference's kernel_flag/kernel_tag). */
 enum {
-  LESPMV_VARIANT_AUTO = 0,      /* the handle's tuned choice */
+  LESPMV_VARIANT_AUTO = 0,      /* the handle's tuned choice; CSR: PANEL when x spans at least 16 MB and a
+                                   third of the entries lie more than 2 MB of x off the diagonal, MERGE when the
+                                   longest row exceeds 32 x max(mean, 4), VECTOR otherwise */
   LESPMV_VARIANT_VECTOR = 1,    /* CSR: lanes-per-row vector kernel; DIA: x through caches */
   LESPMV_VARIANT_MERGE = 2,     /* CSR: merge-path tiles; DIA: LDS-staged x windows */
   LESPMV_VARIANT_ATOMIC = 3,    /* COO: atomics instead of the carry fix-up pass */
@@ -135,9 +142,20 @@ int lespmv_matrix_info(lespmv_matrix_t A, int64_t* m, int64_t* n, int64_t* nnz, 
 
 /* Build format `fmt` on the device from the handle's CSR (GPU kernels). */
 int lespmv_convert(lespmv_matrix_t A, int fmt, const lespmv_params_t* p);
+/* Frees format `fmt`; for LESPMV_CSR the prepared column-panel layout (the CSR arrays stay). */
 int lespmv_release_format(lespmv_matrix_t A, int fmt);
 int lespmv_has_format(lespmv_matrix_t A, int fmt);
-/* Device milliseconds of the last lespmv_convert of `fmt` (hipEvent timed). */
+/* Builds ahead of the first call what kernel `variant` of `fmt` needs beyond
+ * the format's arrays, so that the calls themselves allocate nothing and stay
+ * asynchronous. CSR (LeSpMV_csr, src/spmv_csr.cpp:164-185, keeps a partition
+ * made on first use, :149-153): the row analysis and merge-path partition,
+ * and for a PANEL variant (or AUTO when it picks one) the column-panel layout
+ * (about nnz * (sV + 2.5) + P * M * sV bytes). Other formats: nothing.
+ * Synchronous; lespmv_convert_ms(A, LESPMV_CSR) then gives its device time. A
+ * PANEL call without it builds the layout on its first call (synchronously). */
+int lespmv_prepare(lespmv_matrix_t A, int fmt, int variant);
+/* Device milliseconds of the last lespmv_convert of `fmt` (hipEvent timed);
+ * for LESPMV_CSR the last lespmv_prepare. */
 int lespmv_convert_ms(lespmv_matrix_t A, int fmt, double* ms);
 
 /* y = alpha*A*x + beta*y; x, y are device pointers of the handle's dtype.
@@ -167,7 +185,8 @@ int lespmv_time_spmm(lespmv_matrix_t A, int fmt, int variant, int32_t k, double 
 int lespmv_effective_bytes(lespmv_matrix_t A, int fmt, int32_t k, int beta_nonzero, int64_t* bytes);
 /* The reference's bytes_per_spmv formula (include/spmv_benchmark.h:26-150). */
 int lespmv_reference_bytes(lespmv_matrix_t A, int fmt, int64_t* bytes);
-/* Device bytes held by format `fmt` (LESPMV_CSR: the CSR arrays). */
+/* Device bytes held by format `fmt` (LESPMV_CSR: the CSR arrays plus the
+ * prepared partition and panel layout). */
 int lespmv_format_bytes(lespmv_matrix_t A, int fmt, int64_t* bytes);
 
 /* Read back a format's arrays in the reference's layout (named as the

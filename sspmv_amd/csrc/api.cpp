@@ -14,6 +14,7 @@ thread_local std::string g_last_error;
 
 Matrix::~Matrix() {
   for (auto& f : fmt) f.reset();
+  if (ev_switch) (void)hipEventDestroy(ev_switch);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -41,8 +42,29 @@ void require_format(Matrix& A, int fmt) {
               "format " + std::to_string(fmt) + " has not been converted on this handle");
 }
 
+// The handle's scratch arrays serve one call at a time. A call on another
+// stream than the previous call waits for that call first: after a call on a
+// caller's stream an event is recorded there (the stream is alive then); the
+// handle's own stream, which lives as long as the handle, is marked only when
+// another stream takes over.
+struct StreamOrder {
+  Matrix& A;
+  hipStream_t s;
+  StreamOrder(Matrix& a, hipStream_t st) : A(a), s(st) {
+    if (A.last_stream && A.last_stream != s) {
+      if (A.last_stream == A.stream) LSP_HIP(hipEventRecord(A.ev_switch, A.stream));
+      LSP_HIP(hipStreamWaitEvent(s, A.ev_switch, 0));
+    }
+  }
+  void done() {
+    if (s != A.stream) LSP_HIP(hipEventRecord(A.ev_switch, s));
+    A.last_stream = s;
+  }
+};
+
 void run_spmv(Matrix& A, int fmt, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
   require_format(A, fmt);
+  StreamOrder order(A, s);
   switch (fmt) {
     case LESPMV_CSR: csr_spmv(A, variant, alpha, x, beta, y, s); break;
     case LESPMV_COO: coo_spmv(A, variant, alpha, x, beta, y, s); break;
@@ -55,6 +77,7 @@ void run_spmv(Matrix& A, int fmt, int variant, double alpha, const void* x, doub
     case LESPMV_CSR5: csr5_spmv(A, alpha, x, beta, y, s); break;
     default: throw Error(LESPMV_ERR_ARG, "bad format");
   }
+  order.done();
 }
 
 void run_spmm(Matrix& A, int fmt, int variant, int k, double alpha, const void* X, int64_t ldx, double beta,
@@ -62,6 +85,7 @@ void run_spmm(Matrix& A, int fmt, int variant, int k, double alpha, const void* 
   require_format(A, fmt);
   LSP_REQUIRE(k >= 0 && k <= 64, LESPMV_ERR_UNSUPPORTED, "SpMM supports 0 <= k <= 64");
   LSP_REQUIRE(ldx >= k && ldy >= k, LESPMV_ERR_ARG, "leading dimensions must be >= k");
+  StreamOrder order(A, s);
   switch (fmt) {
     case LESPMV_CSR: csr_spmm(A, variant, k, alpha, X, ldx, beta, Y, ldy, s); break;
     case LESPMV_COO: coo_spmm(A, variant, k, alpha, X, ldx, beta, Y, ldy, s); break;
@@ -73,6 +97,7 @@ void run_spmm(Matrix& A, int fmt, int variant, int k, double alpha, const void* 
     case LESPMV_BSR: bsr_spmm(A, variant, k, alpha, X, ldx, beta, Y, ldy, s); break;
     default: throw Error(LESPMV_ERR_UNSUPPORTED, "SpMM is not provided for this format");
   }
+  order.done();
 }
 
 // Algorithmic bytes of one SpMV/SpMM: values + indices actually describing
@@ -219,6 +244,7 @@ int lespmv_init(int device) {
   return guarded([&] { LSP_HIP(hipSetDevice(device)); });
 }
 
+
 int lespmv_device_info(int64_t* hbm_bytes, int32_t* num_cu, int32_t* arch_gfx) {
   return guarded([&] {
     int dev = 0;
@@ -274,6 +300,7 @@ int lespmv_create_csr(lespmv_matrix_t* out, int dtype, int64_t m, int64_t n, int
     A->nnz = nnz;
     LSP_HIP(hipGetDevice(&A->device));
     LSP_HIP(hipStreamCreate(&A->stream));
+    LSP_HIP(hipEventCreateWithFlags(&A->ev_switch, hipEventDisableTiming));
     LSP_HIP(hipEventCreate(&A->ev0));
     LSP_HIP(hipEventCreate(&A->ev1));
     A->rp.alloc(sizeof(int32_t) * (m + 1));
@@ -295,6 +322,8 @@ int lespmv_destroy(lespmv_matrix_t h) {
     if (!h) return;
     Matrix* A = reinterpret_cast<Matrix*>(h);
     (void)hipStreamSynchronize(A->stream);
+    // the last call on a caller's stream recorded ev_switch there
+    if (A->last_stream && A->last_stream != A->stream) (void)hipEventSynchronize(A->ev_switch);
     delete A;
   });
 }
@@ -340,12 +369,36 @@ int lespmv_convert(lespmv_matrix_t h, int fmt, const lespmv_params_t* prm) {
   });
 }
 
+int lespmv_prepare(lespmv_matrix_t h, int fmt, int variant) {
+  return guarded([&] {
+    Matrix& A = mat(h);
+    check_fmt(fmt);
+    require_format(A, fmt);
+    if (fmt != LESPMV_CSR) return;  // the other formats are complete after lespmv_convert
+    hipStream_t s = A.stream;
+    if (A.last_stream && A.last_stream != A.stream) LSP_HIP(hipEventSynchronize(A.ev_switch));
+    LSP_HIP(hipEventRecord(A.ev0, s));
+    csr_prepare(A, variant, s);
+    LSP_HIP(hipEventRecord(A.ev1, s));
+    LSP_HIP(hipEventSynchronize(A.ev1));
+    float ms = 0;
+    LSP_HIP(hipEventElapsedTime(&ms, A.ev0, A.ev1));
+    A.prepare_ms = ms;
+  });
+}
+
 int lespmv_release_format(lespmv_matrix_t h, int fmt) {
   return guarded([&] {
     Matrix& A = mat(h);
     check_fmt(fmt);
     LSP_HIP(hipStreamSynchronize(A.stream));
-    A.fmt[fmt].reset();
+    if (A.last_stream && A.last_stream != A.stream) LSP_HIP(hipEventSynchronize(A.ev_switch));
+    if (fmt == LESPMV_CSR) {
+      csr_release_aux(A);  // the CSR arrays themselves stay
+      A.prepare_ms = 0;
+    } else {
+      A.fmt[fmt].reset();
+    }
   });
 }
 
@@ -360,7 +413,7 @@ int lespmv_convert_ms(lespmv_matrix_t h, int fmt, double* ms) {
     Matrix& A = mat(h);
     check_fmt(fmt);
     require_format(A, fmt);
-    *ms = fmt == LESPMV_CSR ? 0.0 : A.fmt[fmt]->convert_ms;
+    *ms = fmt == LESPMV_CSR ? A.prepare_ms : A.fmt[fmt]->convert_ms;
   });
 }
 
@@ -440,7 +493,8 @@ int lespmv_format_bytes(lespmv_matrix_t h, int fmt, int64_t* bytes) {
     Matrix& A = mat(h);
     check_fmt(fmt);
     require_format(A, fmt);
-    *bytes = fmt == LESPMV_CSR ? (int64_t)(A.rp.bytes + A.ci.bytes + A.val.bytes) : A.fmt[fmt]->device_bytes();
+    *bytes = fmt == LESPMV_CSR ? (int64_t)(A.rp.bytes + A.ci.bytes + A.val.bytes) + csr_aux_bytes(A)
+                               : A.fmt[fmt]->device_bytes();
   });
 }
 

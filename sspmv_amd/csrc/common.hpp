@@ -161,6 +161,7 @@ struct CsrMeta {
   double avg_row = 0;
   int32_t lanes = 0;         // tuned lanes-per-row of the vector kernel
   int32_t auto_variant = 1;  // tuned CSR kernel
+  double far_frac = 0;       // share of entries more than 2 MB of x away from the diagonal
   int64_t merge_tiles = 0;
   DBuf merge_coord;          // int32[2*(tiles+1)] (row, nnz) at each tile start
   DBuf carry_row;            // int32[tiles]  row open at the tile end (-1: none)
@@ -181,6 +182,12 @@ struct Matrix {
   int device = 0;
   int64_t m = 0, n = 0, nnz = 0;
   hipStream_t stream = nullptr;
+  // The handle's device scratch (merge-path carries, panel partials, fix-up
+  // partials of the other formats) serves one call at a time: a call on
+  // another stream than the previous one first waits for that stream.
+  hipStream_t last_stream = nullptr;
+  hipEvent_t ev_switch = nullptr;
+  double prepare_ms = 0;  // device time of the last CSR prepare (panel layout)
   DBuf rp, ci, val;  // CSR: int32[m+1], int32[nnz], V[nnz]
   CsrMeta meta;
   std::unique_ptr<FormatData> fmt[LESPMV_NUM_FORMATS];
@@ -191,6 +198,20 @@ struct Matrix {
 // --- launchers (one translation unit per format family) ----------------------
 
 void csr_analyse(Matrix& A, hipStream_t s);
+// Builds what a CSR kernel variant needs beyond the CSR arrays (the merge-path
+// partition; the column-panel layout of a PANEL variant). Synchronous.
+void csr_prepare(Matrix& A, int variant, hipStream_t s);
+void csr_release_aux(Matrix& A);
+// Merge-path CSR pieces on plain device pointers: the tile partition (tiles
+// of kMergeItems = 1024 merge-path items, coord int32[2 (tiles + 1)]) and the
+// tiles with their carry fix-up (carry_row int32[tiles], carry_val and
+// head_val V[tiles]).
+constexpr int kMergeItems = 1024;
+void csr_merge_partition(int64_t tiles, int64_t m, int64_t nnz, const int* rp, int* coord, hipStream_t s);
+void csr_merge_tiles(int dtype, int64_t m, int64_t tiles, const int* rp, const int* ci, const void* val,
+                     const void* x, void* y, double alpha, double beta, const int* coord, int* carry_row,
+                     void* carry_val, void* head_val, hipStream_t s);
+int64_t csr_aux_bytes(const Matrix& A);
 void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s);
 void csr_spmm(Matrix& A, int variant, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
               int64_t ldy, hipStream_t s);

@@ -22,9 +22,9 @@ namespace lsp {
 namespace {
 
 constexpr int kBlock = 256;
-// merge-path items (row ends + entries) per tile of the CSR kernel: 1024
-// (stencil 369 against 374 us with 2048, FP32 lognormal 692 against 699)
-constexpr int kMergeItems = 1024;
+// merge-path items (row ends + entries) per tile of the CSR kernel:
+// kMergeItems = 1024 (stencil 369 against 374 us with 2048, FP32 lognormal 692
+// against 699)
 // items per tile of the column-panel kernel: 1024 for FP64 (555 against
 // 566 us on the lognormal matrix: more workgroups fit), 2048 for FP32 (431
 // against 449 us with 1024)
@@ -292,6 +292,7 @@ struct PanelArgs {
   const uint32_t* ends;  // row-end bits of every panel's merge-path items
   const int64_t* endoff; // [P] first word of each panel's bits
   int64_t width;         // columns per panel
+  int64_t n;             // columns of the matrix
   const uint16_t* ci16;  // compact columns (panel-local): low 16 bits
   const uint8_t* cihi;   // and high nibbles, two entries per byte
 };
@@ -310,9 +311,12 @@ __global__ __launch_bounds__(kBlock) void panel_merge_kernel(int64_t m, PanelArg
   if (p >= a.P || t >= a.tiles[p]) return;
   const int64_t off = a.nzoff[p];
   // compact columns are panel-local: x starts at the panel's first column
-  const V* xs = COMPACT ? x + (int64_t)p * a.width : x;
+  // (a panel past the last column has no entries; its base is clamped to n)
+  const int64_t first = (int64_t)p * a.width < a.n ? (int64_t)p * a.width : a.n;
+  const V* xs = COMPACT ? x + first : x;
   merge_tile<V, true, kPanelItems<V>, true, COMPACT>(
-      (int)t, m, a.prp + (int64_t)p * (m + 1), ci + off, val + off, xs, partial + (int64_t)p * m, V(1), V(0),
+      (int)t, m, a.prp + (int64_t)p * (m + 1), COMPACT ? nullptr : ci + off, val + off, xs, partial + (int64_t)p * m,
+      V(1), V(0),
       a.coord + (int64_t)p * 2 * (a.maxT + 1), a.carry_row + (int64_t)p * a.maxT, carry_val + (int64_t)p * a.maxT,
       head_val + (int64_t)p * a.maxT, a.ends + a.endoff[p], a.ci16, a.cihi, off);
 }
@@ -482,20 +486,27 @@ void vector_dispatch(Matrix& A, int lanes, double alpha, const void* x, double b
   }
 }
 
+// The merge-path tiles and their carry fix-up over a partition `coord` of
+// `tiles` tiles; carry_row / carry_val / head_val hold one entry per tile.
+template <class V>
+void merge_launch(int64_t m, int64_t tiles, const int* rp, const int* ci, const V* val, const void* x, void* y,
+                  double alpha, double beta, const int* coord, int* carry_row, V* carry_val, V* head_val,
+                  hipStream_t s) {
+  if (tiles == 0) return;
+  hipLaunchKernelGGL((csr_merge_kernel<V>), dim3((unsigned)tiles), dim3(kBlock), 0, s, m, rp, ci, val,
+                     static_cast<const V*>(x), static_cast<V*>(y), (V)alpha, (V)beta, coord, carry_row, carry_val,
+                     head_val);
+  LSP_LAUNCHED();
+  hipLaunchKernelGGL((csr_merge_fixup_kernel<V>), grid1d(tiles, kBlock), dim3(kBlock), 0, s, tiles, coord, rp,
+                     carry_row, carry_val, head_val, static_cast<V*>(y), (V)alpha, (V)beta);
+  LSP_LAUNCHED();
+}
+
 template <class V>
 void merge_run(Matrix& A, double alpha, const void* x, double beta, void* y, hipStream_t s) {
   CsrMeta& mt = A.meta;
-  const int64_t tiles = mt.merge_tiles;
-  if (tiles == 0) return;
-  hipLaunchKernelGGL((csr_merge_kernel<V>), dim3((unsigned)tiles), dim3(kBlock), 0, s, A.m, A.rp.as<int>(),
-                     A.ci.as<int>(), A.val.as<V>(), static_cast<const V*>(x), static_cast<V*>(y), (V)alpha,
-                     (V)beta, mt.merge_coord.as<int>(), mt.carry_row.as<int>(), mt.carry_val.as<V>(),
-                     mt.head_val.as<V>());
-  LSP_LAUNCHED();
-  hipLaunchKernelGGL((csr_merge_fixup_kernel<V>), grid1d(tiles, kBlock), dim3(kBlock), 0, s, tiles,
-                     mt.merge_coord.as<int>(), A.rp.as<int>(), mt.carry_row.as<int>(), mt.carry_val.as<V>(),
-                     mt.head_val.as<V>(), static_cast<V*>(y), (V)alpha, (V)beta);
-  LSP_LAUNCHED();
+  merge_launch<V>(A.m, mt.merge_tiles, A.rp.as<int>(), A.ci.as<int>(), A.val.as<V>(), x, y, alpha, beta,
+                  mt.merge_coord.as<int>(), mt.carry_row.as<int>(), mt.carry_val.as<V>(), mt.head_val.as<V>(), s);
 }
 
 // Splits the matrix into P column panels (row order kept inside each panel).
@@ -559,6 +570,8 @@ void build_panels(Matrix& A, int P, hipStream_t s) {
   // entry they save (398 against 393 us), so FP32 keeps 32-bit columns.
   mt.p_width = width;
   mt.p_compact = sizeof(V) == 8 && width <= (int64_t{1} << 20);
+  mt.p_ci16.reset();
+  mt.p_cihi.reset();
   if (mt.p_compact) {
     mt.p_ci16.alloc(sizeof(uint16_t) * std::max<int64_t>(A.nnz, 1));
     mt.p_cihi.alloc(std::max<int64_t>((A.nnz + 1) / 2, 1));
@@ -581,7 +594,7 @@ void build_panels(Matrix& A, int P, hipStream_t s) {
 template <class V>
 void panel_run(Matrix& A, int P, double alpha, const void* x, double beta, void* y, hipStream_t s) {
   CsrMeta& mt = A.meta;
-  if (mt.panels != P) build_panels<V>(A, P, s);
+  if (mt.panels != P) build_panels<V>(A, P, s);  // first call without csr_prepare
   const int share = P < kXcds ? kXcds / P : 1;
   PanelArgs a{P,
               share,
@@ -594,6 +607,7 @@ void panel_run(Matrix& A, int P, double alpha, const void* x, double beta, void*
               mt.p_ends.as<uint32_t>(),
               mt.p_endoff.as<int64_t>(),
               mt.p_width,
+              A.n,
               mt.p_ci16.as<uint16_t>(),
               mt.p_cihi.as<uint8_t>()};
   const int64_t blocks = share > 1 ? kXcds * ceil_div(mt.p_maxT, share) : P * mt.p_maxT;
@@ -677,6 +691,35 @@ void spmm_dispatch(Matrix& A, int k, double alpha, const void* X, int64_t ldx, d
   else launch_spmm<V, 64>(A, k, alpha, X, ldx, beta, Y, ldy, s);
 }
 
+// Entries whose column lies more than `window` columns from the row's
+// diagonal position r * n / m (counted once, for the kernel choice).
+__global__ __launch_bounds__(kBlock) void far_count_kernel(int64_t m, int64_t n, int64_t window,
+                                                           const int* __restrict__ rp, const int* __restrict__ ci,
+                                                           unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[kBlock / 64];
+  unsigned long long c = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < m; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = r * n / m;
+    for (int j = rp[r]; j < rp[r + 1]; j++) {
+      const int64_t dc = ci[j] - d;
+      c += (dc > window || dc < -window) ? 1ull : 0ull;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / 64; i++) t += red[i];
+    if (t) atomicAdd(out, t);
+  }
+}
+
+int panels_of(int variant) {
+  return variant == LESPMV_VARIANT_PANEL ? 8 : variant == LESPMV_VARIANT_PANEL16 ? 16
+                                                : variant == LESPMV_VARIANT_PANEL4 ? 4 : 0;
+}
+
 // Lanes per row of the vector kernel: each lane keeps four loads in flight,
 // so uniform rows want about avg/4 lanes; long-tailed rows want about avg
 // lanes so a wave's rows finish together.
@@ -690,16 +733,56 @@ int lanes_for(double avg, int64_t max_row) {
 
 }  // namespace
 
+void csr_merge_partition(int64_t tiles, int64_t m, int64_t nnz, const int* rp, int* coord, hipStream_t s) {
+  hipLaunchKernelGGL(merge_partition_kernel, grid1d(tiles + 1, kBlock), dim3(kBlock), 0, s, tiles, m, nnz, rp, coord,
+                     (int64_t)kMergeItems);
+  LSP_LAUNCHED();
+}
+
+void csr_merge_tiles(int dtype, int64_t m, int64_t tiles, const int* rp, const int* ci, const void* val,
+                     const void* x, void* y, double alpha, double beta, const int* coord, int* carry_row,
+                     void* carry_val, void* head_val, hipStream_t s) {
+  if (dtype == LESPMV_F64)
+    merge_launch<double>(m, tiles, rp, ci, static_cast<const double*>(val), x, y, alpha, beta, coord, carry_row,
+                         static_cast<double*>(carry_val), static_cast<double*>(head_val), s);
+  else
+    merge_launch<float>(m, tiles, rp, ci, static_cast<const float*>(val), x, y, alpha, beta, coord, carry_row,
+                        static_cast<float*>(carry_val), static_cast<float*>(head_val), s);
+}
+
 void csr_analyse(Matrix& A, hipStream_t s) {
   CsrMeta& mt = A.meta;
   mt.max_row = A.m ? reduce_max_rowlen(A.rp.as<int32_t>(), A.m, s) : 0;
   mt.avg_row = A.m ? (double)A.nnz / (double)A.m : 0.0;
   mt.lanes = lanes_for(mt.avg_row, mt.max_row);
-  // the merge path bounds every workgroup's work; it is the default only when
-  // the longest row is far beyond the mean (timings on the selector's
-  // synthetic families favour the vector kernel below that)
-  mt.auto_variant = mt.max_row > 32 * std::max<double>(mt.avg_row, 4.0) ? LESPMV_VARIANT_MERGE
-                                                                        : LESPMV_VARIANT_VECTOR;
+  // share of entries gathered from x more than 2 MB away from the diagonal
+  mt.far_frac = 0.0;
+  if (A.m && A.nnz) {
+    DBuf cnt;
+    cnt.alloc(sizeof(unsigned long long));
+    LSP_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
+    const int64_t window = (int64_t{2} << 20) / (int64_t)vsize(A.dtype);
+    hipLaunchKernelGGL(far_count_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(A.m, kBlock), 4096)), dim3(kBlock),
+                       0, s, A.m, A.n, window, A.rp.as<int>(), A.ci.as<int>(), cnt.as<unsigned long long>());
+    LSP_LAUNCHED();
+    unsigned long long far = 0;
+    LSP_HIP(hipMemcpyAsync(&far, cnt.p, sizeof(far), hipMemcpyDeviceToHost, s));
+    LSP_HIP(hipStreamSynchronize(s));
+    mt.far_frac = (double)far / (double)A.nnz;
+  }
+  // Column panels when x is at least four times an XCD's 4 MB L2 and a third
+  // of the entries gather from far off the diagonal: every XCD then keeps its
+  // own slice of x (lognormal matrix of 4M columns, 70 % uniform columns:
+  // FP64 531 us against 821 us for the vector kernel, FP32 396 against 694).
+  // Otherwise the merge path, which bounds every workgroup's work, when the
+  // longest row is far beyond the mean (timings on the selector's synthetic
+  // families favour the vector kernel below that).
+  const double xbytes = (double)A.n * (double)vsize(A.dtype);
+  if (mt.far_frac >= 0.3 && xbytes >= 16.0 * (1 << 20))
+    mt.auto_variant = LESPMV_VARIANT_PANEL;
+  else
+    mt.auto_variant = mt.max_row > 32 * std::max<double>(mt.avg_row, 4.0) ? LESPMV_VARIANT_MERGE
+                                                                          : LESPMV_VARIANT_VECTOR;
   const int64_t items = A.m + A.nnz;
   mt.merge_tiles = ceil_div(items, kMergeItems);
   mt.merge_coord.alloc(sizeof(int) * 2 * (mt.merge_tiles + 1));
@@ -709,7 +792,37 @@ void csr_analyse(Matrix& A, hipStream_t s) {
   hipLaunchKernelGGL(merge_partition_kernel, grid1d(mt.merge_tiles + 1, kBlock), dim3(kBlock), 0, s, mt.merge_tiles,
                      A.m, A.nnz, A.rp.as<int>(), mt.merge_coord.as<int>(), (int64_t)kMergeItems);
   LSP_LAUNCHED();
+  // the partition is complete before any call (on any stream) reads it
+  LSP_HIP(hipStreamSynchronize(s));
   mt.analysed = true;
+}
+
+void csr_prepare(Matrix& A, int variant, hipStream_t s) {
+  if (!A.meta.analysed) csr_analyse(A, s);
+  if (variant == LESPMV_VARIANT_AUTO) variant = A.meta.auto_variant;
+  const int P = panels_of(variant);
+  if (P && A.m && A.meta.panels != P) {
+    if (A.dtype == LESPMV_F64) build_panels<double>(A, P, s);
+    else build_panels<float>(A, P, s);
+  }
+}
+
+void csr_release_aux(Matrix& A) {
+  CsrMeta& mt = A.meta;
+  for (DBuf* b : {&mt.p_nzoff, &mt.p_rp, &mt.p_ci, &mt.p_val, &mt.p_tiles, &mt.p_coord, &mt.p_carry_row,
+                  &mt.p_carry_val, &mt.p_head, &mt.p_partial, &mt.p_ends, &mt.p_endoff, &mt.p_ci16, &mt.p_cihi})
+    b->reset();
+  mt.panels = 0;
+}
+
+int64_t csr_aux_bytes(const Matrix& A) {
+  const CsrMeta& mt = A.meta;
+  int64_t b = 0;
+  for (const DBuf* d : {&mt.merge_coord, &mt.carry_row, &mt.carry_val, &mt.head_val, &mt.p_nzoff, &mt.p_rp,
+                        &mt.p_ci, &mt.p_val, &mt.p_tiles, &mt.p_coord, &mt.p_carry_row, &mt.p_carry_val, &mt.p_head,
+                        &mt.p_partial, &mt.p_ends, &mt.p_endoff, &mt.p_ci16, &mt.p_cihi})
+    b += (int64_t)d->bytes;
+  return b;
 }
 
 void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
@@ -721,8 +834,7 @@ void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, 
     else merge_run<float>(A, alpha, x, beta, y, s);
     return;
   }
-  if (variant == LESPMV_VARIANT_PANEL || variant == LESPMV_VARIANT_PANEL16 || variant == LESPMV_VARIANT_PANEL4) {
-    const int P = variant == LESPMV_VARIANT_PANEL ? 8 : variant == LESPMV_VARIANT_PANEL16 ? 16 : 4;
+  if (const int P = panels_of(variant)) {
     if (A.dtype == LESPMV_F64) panel_run<double>(A, P, alpha, x, beta, y, s);
     else panel_run<float>(A, P, alpha, x, beta, y, s);
     return;

@@ -125,6 +125,7 @@ def load() -> C.CDLL:
         "lespmv_release_format": ([vp, C.c_int], C.c_int),
         "lespmv_has_format": ([vp, C.c_int], C.c_int),
         "lespmv_convert_ms": ([vp, C.c_int, C.POINTER(dbl)], C.c_int),
+        "lespmv_prepare": ([vp, C.c_int, C.c_int], C.c_int),
         "lespmv_spmv": ([vp, C.c_int, C.c_int, dbl, vp, dbl, vp, vp], C.c_int),
         "lespmv_spmm": ([vp, C.c_int, C.c_int, i32, dbl, vp, i64, dbl, vp, i64, vp], C.c_int),
         "lespmv_time_spmv": ([vp, C.c_int, C.c_int, dbl, vp, dbl, vp, i32, C.POINTER(dbl)], C.c_int),
@@ -367,6 +368,12 @@ class Matrix:
 
     def release(self, fmt: int):
         check(load().lespmv_release_format(self.h, fmt))
+
+    def prepare(self, fmt: int, variant: int = 0) -> float:
+        """Builds what `variant` needs ahead of the first call (CSR: the merge-path
+        partition, the column-panel layout); returns its device ms."""
+        check(load().lespmv_prepare(self.h, fmt, variant))
+        return self.convert_ms(fmt)
 
     def convert_ms(self, fmt: int) -> float:
         ms = C.c_double()

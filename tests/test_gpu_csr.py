@@ -194,3 +194,88 @@ def test_csr_panels_wide_and_compact_columns(gpu, oracle, dt):
     for var in (gpu.VARIANT_PANEL16, gpu.VARIANT_PANEL, gpu.VARIANT_PANEL4):
         got = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
         assert rel2(got, ref) <= TOL[np.dtype(dt)], (np.dtype(dt).name, var, rel2(got, ref))
+
+
+def test_scratch_calls_across_streams(gpu, oracle):
+    """A fresh handle's MERGE and PANEL calls (lazily analysed, layout built on
+    first use) on a caller's stream, then on the handle's own stream, then back:
+    each call's y against the oracle. The handle's scratch (partition, carries,
+    panel partials) must be complete and not shared by overlapping calls."""
+    import ctypes as C
+    rng = np.random.default_rng(77)
+    A = gpu.generate(gpu.GEN_LOGNORMAL, [150000, 150000, 16.0, 1.0, 4096, 0.3, 64], 13)
+    x = rng.uniform(-1, 1, A.n)
+    y0 = rng.uniform(-1, 1, A.m)
+    ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+    L = gpu.load()
+    s = C.c_void_p()
+    gpu.check(L.lespmv_stream_create(C.byref(s)))
+    try:
+        for var in (gpu.VARIANT_MERGE, gpu.VARIANT_PANEL, gpu.VARIANT_PANEL4):
+            M = gpu.Matrix.from_host(A)
+            dx = gpu.DeviceArray.from_numpy(x)
+            ys = [gpu.DeviceArray.from_numpy(y0) for _ in range(3)]
+            M.spmv(CSR, 0.8, dx, 0.7, ys[0], variant=var, stream=s)   # caller's stream first
+            M.spmv(CSR, 0.8, dx, 0.7, ys[1], variant=var)             # then the handle's own
+            M.spmv(CSR, 0.8, dx, 0.7, ys[2], variant=var, stream=s)   # and back
+            gpu.check(L.lespmv_stream_synchronize(s))
+            gpu.synchronize()
+            for i, dy in enumerate(ys):
+                err = rel2(dy.numpy(), ref)
+                assert err <= 1e-12, (var, i, err)
+    finally:
+        L.lespmv_stream_destroy(s)
+
+
+def test_panels_compact_column_boundaries(gpu, oracle):
+    """FP64 panels of exactly 2^20 columns (4 panels of a 4,194,304-column
+    matrix) and of 2^19 (8 panels) keep panel-local columns in 16 bits plus a
+    nibble; entries at each panel's first column p * 2^20 and last column
+    p * 2^20 + 2^20 - 1 (local 0xFFFFF) must gather the right x."""
+    n = 4194304
+    rng = np.random.default_rng(9)
+    rows = []
+    for p in range(4):
+        rows.append([p << 20, (p << 20) + (1 << 20) - 1])
+        rows.append([(p << 20) + (1 << 19) - 1, (p << 20) + (1 << 19)])
+    rows.append(list(range(0, n, 1 << 16)))
+    rows.append([n - 1])
+    rows.append([])
+    m = len(rows)
+    rp = np.zeros(m + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate([np.array(r, np.int32) for r in rows if r]).astype(np.int32)
+    A = HostCsr(m, n, rp, ci, rng.uniform(-1, 1, len(ci)))
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, m)
+    ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+    M = gpu.Matrix.from_host(A)
+    for var in (gpu.VARIANT_PANEL4, gpu.VARIANT_PANEL, gpu.VARIANT_PANEL16):
+        M.prepare(CSR, var)
+        got = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
+        assert rel2(got, ref) <= 1e-12, (var, rel2(got, ref))
+
+
+def test_auto_picks_panels_on_scattered_columns(gpu, oracle):
+    """AUTO takes the column panels on a matrix whose x spans 32 MB (FP64) with
+    most columns far off the diagonal, and the vector / merge kernels on banded
+    ones; lespmv_prepare builds the layout ahead (its time and bytes reported)
+    and lespmv_release_format(CSR) frees it again."""
+    A = gpu.generate(gpu.GEN_LOGNORMAL, [4194304, 4194304, 2.0, 1.0, 64, 0.3, 64], 17)
+    M = gpu.Matrix.from_host(A)
+    base = M.format_bytes(CSR)
+    ms = M.prepare(CSR, gpu.VARIANT_AUTO)
+    grown = M.format_bytes(CSR)
+    print(f"prepare (AUTO) {ms:.2f} ms, CSR bytes {base} -> {grown}")
+    assert ms > 0 and grown > base + A.nnz * 8  # the panel layout exists
+    x = gpu.fill_uniform(A.n, 4)
+    y0 = gpu.fill_uniform(A.m, 9)
+    ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+    assert rel2(gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, gpu.VARIANT_AUTO), ref) <= 1e-12
+    M.release(CSR)
+    assert M.format_bytes(CSR) < grown
+    assert rel2(gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, gpu.VARIANT_AUTO), ref) <= 1e-12  # rebuilt on first use
+    B = gpu.generate(gpu.GEN_LAPLACE2D, [2048, 2048, 4.0, -1.0], 0)
+    MB = gpu.Matrix.from_host(B)
+    MB.prepare(CSR, gpu.VARIANT_AUTO)
+    assert MB.format_bytes(CSR) < B.nnz * 12 + (B.m + 1) * 4 + 64 * 1024 * 1024  # no panels for a banded matrix
