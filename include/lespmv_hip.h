@@ -179,6 +179,25 @@ int lespmv_time_spmv(lespmv_matrix_t A, int fmt, int variant, double alpha, cons
 int lespmv_time_spmm(lespmv_matrix_t A, int fmt, int variant, int32_t k, double alpha, const void* X, int64_t ldx,
                      double beta, void* Y, int64_t ldy, int32_t iters, double* ms_per_call);
 
+/* Allocation-free CSR SpMV. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). y = alpha*A*x + beta*y with
+ * the merge-path kernels of LESPMV_VARIANT_MERGE on the caller's device CSR
+ * arrays (row_offset[m+1], col_index[nnz], values[nnz]) and x, y; bitwise the
+ * result of lespmv_spmv(LESPMV_CSR, LESPMV_VARIANT_MERGE) on the same matrix.
+ * Calls no allocator: the tile partition and carries live in a fixed device
+ * array of 3,670,272 bytes per GPU (held until the process ends), which limits
+ * m + nnz to lespmv_csr_noalloc_max_items() = 134,217,728 merge-path items;
+ * larger inputs return LESPMV_ERR_UNSUPPORTED. stream NULL = the null stream.
+ * Every call on one GPU shares that array: calls that overlap in time on one
+ * GPU (other streams, other threads) race and must be serialised by the
+ * caller. lespmv_init(device) looks the array up on that device; otherwise the
+ * first call on a device does. */
+int lespmv_csr_spmv_noalloc(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* row_offset,
+                            const int32_t* col_index, const void* values, double alpha, const void* x, double beta,
+                            void* y, lespmv_stream_t stream);
+int64_t lespmv_csr_noalloc_max_items(void);
+
 /* Algorithmic bytes of one SpMV (k = 1) or SpMM with this format: values +
  * indices + x read once + y written (+ y read if beta != 0). Padding and
  * repeated x gathers excluded. */

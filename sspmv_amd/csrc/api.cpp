@@ -241,7 +241,10 @@ extern "C" {
 const char* lespmv_last_error(void) { return g_last_error.c_str(); }
 
 int lespmv_init(int device) {
-  return guarded([&] { LSP_HIP(hipSetDevice(device)); });
+  return guarded([&] {
+    LSP_HIP(hipSetDevice(device));
+    (void)noalloc_scratch(device);  // the allocation-free SpMV's scratch address on this device
+  });
 }
 
 

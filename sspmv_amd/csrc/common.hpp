@@ -211,6 +211,9 @@ void csr_merge_partition(int64_t tiles, int64_t m, int64_t nnz, const int* rp, i
 void csr_merge_tiles(int dtype, int64_t m, int64_t tiles, const int* rp, const int* ci, const void* val,
                      const void* x, void* y, double alpha, double beta, const int* coord, int* carry_row,
                      void* carry_val, void* head_val, hipStream_t s);
+// Device address of the allocation-free SpMV's fixed scratch on `device`
+// (noalloc.hip), looked up on first use with that device current.
+void* noalloc_scratch(int device);
 int64_t csr_aux_bytes(const Matrix& A);
 void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s);
 void csr_spmm(Matrix& A, int variant, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,

@@ -126,6 +126,8 @@ def load() -> C.CDLL:
         "lespmv_has_format": ([vp, C.c_int], C.c_int),
         "lespmv_convert_ms": ([vp, C.c_int, C.POINTER(dbl)], C.c_int),
         "lespmv_prepare": ([vp, C.c_int, C.c_int], C.c_int),
+        "lespmv_csr_spmv_noalloc": ([C.c_int, i64, i64, i64, vp, vp, vp, dbl, vp, dbl, vp, vp], C.c_int),
+        "lespmv_csr_noalloc_max_items": ([], i64),
         "lespmv_spmv": ([vp, C.c_int, C.c_int, dbl, vp, dbl, vp, vp], C.c_int),
         "lespmv_spmm": ([vp, C.c_int, C.c_int, i32, dbl, vp, i64, dbl, vp, i64, vp], C.c_int),
         "lespmv_time_spmv": ([vp, C.c_int, C.c_int, dbl, vp, dbl, vp, i32, C.POINTER(dbl)], C.c_int),
