@@ -6,8 +6,13 @@ Workload (one matrix per section, FP64 unless stated, int32 indices):
     capped at 4096), 70 % of columns uniform over [0, N) and 30 % within +-64 of
     the diagonal, values U[-1,1), x = U[-1,1) (seed 4); FP64 (headline) and FP32;
   * stencil27: 27-point stencil on a 160^3 grid (diag 26, off-diagonals -1) for
-    the DIA and BSR paths.
-One step = one y = A*x with the matrix and vectors resident in HBM.
+    the DIA and BSR paths, FP64 and FP32;
+  * laplace2d: the 5-point Laplacian on a 1024^2 grid (FP64), timed rotating
+    over four device copies so that every call reads HBM, with the reference's
+    CPU kernel on the same matrix beside it.
+One step = one y = A*x with the matrix and vectors resident in HBM. Every
+kernel's y is compared with the reference's FP64 CPU result on the same
+inputs (rel2_vs_fp64_reference; FP32 kernels against the FP64 result).
 
 Prints ONE JSON line. `value` is the effective bandwidth (algorithmic bytes:
 values + indices + x read once + y written) of the fastest kernel (least time)
@@ -28,6 +33,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -47,19 +53,25 @@ def log(*a):
 def kernels(lib, section):
     """(label, fmt, variant, params) measured per section."""
     sell = lambda C, s: lib.make_params(chunk=C, sigma=s, alignment=1)
-    if section == "stencil27":
-        return [("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None), ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None),
-                ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0,
-                                                                    sell(64, 4096)),
-                ("DIA", lib.DIA, lib.VARIANT_VECTOR, lib.make_params(alignment=1, max_diags=10240)),
-                ("DIA-lds", lib.DIA, lib.VARIANT_MERGE, lib.make_params(alignment=1, max_diags=10240)),
-                ("BSR-4x4", lib.BSR, 0, lib.make_params(block_rows=4, block_cols=4))]
-    ks = [("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None), ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None),
-          ("CSR-panel8", lib.CSR, lib.VARIANT_PANEL, None), ("CSR-panel16", lib.CSR, lib.VARIANT_PANEL16, None),
-          ("CSR-panel4", lib.CSR, lib.VARIANT_PANEL4, None),
-          ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, sell(64, 4096)), ("SELL-64-R", lib.SELL_C_R, 0, sell(64, 0)),
-          ("COO", lib.COO, 0, None), ("CSR5-64x16", lib.CSR5, 0, lib.make_params(omega=64, sigma=16))]
-    return ks if section == "lognormal" else ks[:6]
+    csr = [("CSR-auto", lib.CSR, lib.VARIANT_AUTO, None), ("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None),
+           ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None)]
+    if section.startswith("stencil27"):
+        return csr + [("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, sell(64, 4096)),
+                      ("DIA", lib.DIA, lib.VARIANT_VECTOR, lib.make_params(alignment=1, max_diags=10240)),
+                      ("DIA-lds", lib.DIA, lib.VARIANT_MERGE, lib.make_params(alignment=1, max_diags=10240)),
+                      ("BSR-4x4", lib.BSR, 0, lib.make_params(block_rows=4, block_cols=4))]
+    if section == "laplace2d":
+        return csr + [("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, sell(64, 4096)),
+                      ("DIA", lib.DIA, lib.VARIANT_VECTOR, lib.make_params(alignment=1, max_diags=10240))]
+    # lognormal (FP64 and FP32): ELL and S-ELL are refused beyond 8 x nnz stored slots (the refusal
+    # carries the padded size); DIA does not apply (more than 10,240 diagonals)
+    return csr + [("CSR-panel8", lib.CSR, lib.VARIANT_PANEL, None), ("CSR-panel16", lib.CSR, lib.VARIANT_PANEL16, None),
+                  ("CSR-panel4", lib.CSR, lib.VARIANT_PANEL4, None),
+                  ("ELL", lib.ELL, 0, lib.make_params(chunk=64, alignment=1, max_fill=8)),
+                  ("S-ELL-64", lib.SELL, 0, lib.make_params(chunk=64, sigma=1, alignment=1, max_fill=8)),
+                  ("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, sell(64, 4096)), ("SELL-64-R", lib.SELL_C_R, 0,
+                                                                                sell(64, 0)),
+                  ("COO", lib.COO, 0, None), ("CSR5-64x16", lib.CSR5, 0, lib.make_params(omega=64, sigma=16))]
 
 
 def traffic_from_profiles(label):
@@ -74,17 +86,37 @@ def traffic_from_profiles(label):
         return None
 
 
-def measure(lib, M, dx, dy, section, wanted, warmup):
-    results = []
+def rel2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb else 1.0))
+
+
+def measure(lib, M, dx, dy, section, wanted, warmup, yref=None):
+    """Each kernel of the section: convert (or prepare the CSR variant's layout),
+    time 50 calls with hipEvents, and (with yref, the FP64 reference y) the
+    relative 2-norm error of its y. A conversion the library refuses is recorded
+    with its reason (for ELL / S-ELL the padded slot count)."""
+    results, refused = [], {}
     for label, fmt, variant, params in kernels(lib, section):
         if wanted and label not in wanted:
             continue
         try:
             if fmt != lib.CSR:
                 M.convert(fmt, params)
-            conv_ms = M.convert_ms(fmt) if fmt != lib.CSR else 0.0
+                conv_ms = M.convert_ms(fmt)
+            else:
+                conv_ms = M.prepare(fmt, variant)  # partition / column-panel layout, timed
         except lib.LespmvError as e:
-            log(f"[{section}] {label}: skipped ({e})")
+            msg = str(e)
+            log(f"[{section}] {label}: refused ({msg})")
+            slots = re.search(r"(\d+) stored slots", msg)
+            refused[label] = {"refused": msg, "code": e.code}
+            if slots:
+                vb = 4 if section.endswith("fp32") else 8
+                refused[label]["padded_slots"] = int(slots.group(1))
+                refused[label]["padded_bytes"] = int(slots.group(1)) * (4 + vb)
             continue
         eff = M.effective_bytes(fmt, 1, False)
         try:  # the reference's own formula (include/spmv_benchmark.h:26-160), for comparison only
@@ -93,21 +125,101 @@ def measure(lib, M, dx, dy, section, wanted, warmup):
             ref_bytes = None
         M.time_spmv(fmt, 1.0, dx, 0.0, dy, max(warmup, 1), variant)
         ms = M.time_spmv(fmt, 1.0, dx, 0.0, dy, 50, variant)
+        err = rel2(dy.numpy(), yref) if yref is not None else None  # y = A x of the last timed call
         results.append(dict(label=label, fmt=fmt, variant=variant, params=params, ms=ms, gbs=eff / ms / 1e6,
                             effective_bytes=eff, convert_ms=conv_ms, format_bytes=M.format_bytes(fmt),
-                            ref_gbs=ref_bytes / ms / 1e6 if ref_bytes else None))
-        log(f"[{section}] {label}: {ms * 1e3:.1f} us  {eff / ms / 1e6:.0f} GB/s  convert {conv_ms:.1f} ms")
+                            ref_gbs=ref_bytes / ms / 1e6 if ref_bytes else None, rel2=err))
+        log(f"[{section}] {label}: {ms * 1e3:.1f} us  {eff / ms / 1e6:.0f} GB/s  convert {conv_ms:.1f} ms"
+            + (f"  rel2 {err:.2e}" if err is not None else ""))
+        M.release(fmt)  # CSR: frees the panel layout (the CSR arrays stay)
+    return results, refused
+
+
+def summary(results, refused=None):
+    out = {r["label"]: {"us": round(r["ms"] * 1e3, 2), "GBs": round(r["gbs"], 1),
+                        "frac": round(r["gbs"] * 1e9 / HBM_PEAK, 4), "convert_ms": round(r["convert_ms"], 2),
+                        "format_bytes": r["format_bytes"],
+                        "GBs_reference_formula": round(r["ref_gbs"], 1) if r["ref_gbs"] else None,
+                        **({"rel2_vs_fp64_reference": r["rel2"]} if r.get("rel2") is not None else {})}
+           for r in results}
+    out.update(refused or {})
+    return out
+
+
+def host_info():
+    """CPU model and topology of this host (lscpu), and the OpenMP thread setting."""
+    info = {"omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except Exception as e:  # lscpu absent: keep what /proc/cpuinfo says
+        info["lscpu_error"] = str(e)
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return info
+
+
+def laplace_section(lib, steps, warmup, cpu, cpu_seconds, threads, copies=4):
+    """Config 1: 5-point Laplacian on a 1024 x 1024 grid (N = 1,048,576,
+    nnz = 5,238,784, diag 4, off-diagonal -1), FP64, x = U[-1,1) seed 1. One
+    matrix moves 83.84 MB per SpMV, which the 256 MB Infinity Cache would hold,
+    so the HBM figure rotates over `copies` device copies (matrix, x and y each
+    their own, 335 MB together); the single-copy (cache-resident) time is
+    reported beside it. The CPU side is the reference's LeSpMV_csr
+    kernel_flag=2 (benchmark_spmv_csr.cpp:109 -> test_spmv_csr.cpp:14-87) on
+    the same matrix, on this host's threads."""
+    A = lib.generate(lib.GEN_LAPLACE2D, [1024, 1024, 4.0, -1.0], 0, np.float64)
+    x = lib.fill_uniform(A.n, 1, np.float64)
+    mats = [lib.Matrix.from_host(A) for _ in range(copies)]
+    xs = [lib.DeviceArray.from_numpy(x) for _ in range(copies)]
+    ys = [lib.DeviceArray.empty(A.m, np.float64) for _ in range(copies)]
+    out = {"workload": f"5-point Laplacian 1024x1024 (N={A.m}, nnz={A.nnz}), FP64, int32 indices",
+           "rotation_copies": copies, "kernels": {}}
+    yref = None
+    if cpu:
+        ms, iters, y_cpu, kind, what = cpu_baseline(A, x, threads, cpu_seconds)
+        yref = y_cpu
+        eff = mats[0].effective_bytes(lib.CSR)
+        refb = mats[0].reference_bytes(lib.CSR)
+        out["cpu"] = {"ms": round(ms, 5), "GFLOPs": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2),
+                      "GBs": round(eff / (ms * 1e-3) / 1e9, 2), "GBs_reference_formula": round(refb / (ms * 1e-3) / 1e9, 2),
+                      "iterations": iters, "threads": threads, "kind": kind, "kernel": what, "host": host_info()}
+        log(f"[laplace2d] CPU {what}: {ms:.3f} ms  {eff / (ms * 1e-3) / 1e9:.1f} GB/s")
+    for label, fmt, variant, params in kernels(lib, "laplace2d"):
         if fmt != lib.CSR:
+            for M in mats:
+                M.convert(fmt, params)
+        M0 = mats[0]
+        eff = M0.effective_bytes(fmt, 1, False)
+        lib.time_spmv_rotating(mats, fmt, 1.0, xs, 0.0, ys, max(1, warmup // 4), variant)
+        ms_hbm = lib.time_spmv_rotating(mats, fmt, 1.0, xs, 0.0, ys, steps, variant)
+        ms_cache = M0.time_spmv(fmt, 1.0, xs[0], 0.0, ys[0], steps, variant)
+        row = {"us": round(ms_hbm * 1e3, 2), "GBs": round(eff / ms_hbm / 1e6, 1),
+               "frac": round(eff / ms_hbm / 1e6 * 1e9 / HBM_PEAK, 4),
+               "us_cache_resident": round(ms_cache * 1e3, 2), "GBs_cache_resident": round(eff / ms_cache / 1e6, 1)}
+        if yref is not None:
+            row["rel2_vs_fp64_reference"] = rel2(ys[0].numpy(), yref)
+        out["kernels"][label] = row
+        log(f"[laplace2d] {label}: {ms_hbm * 1e3:.1f} us rotating ({eff / ms_hbm / 1e6:.0f} GB/s), "
+            f"{ms_cache * 1e3:.1f} us one copy")
+        for M in mats:
             M.release(fmt)
-    return results
-
-
-def summary(results):
-    return {r["label"]: {"us": round(r["ms"] * 1e3, 2), "GBs": round(r["gbs"], 1), "frac": round(r["gbs"] * 1e9 /
-                                                                                                  HBM_PEAK, 4),
-                         "convert_ms": round(r["convert_ms"], 2), "format_bytes": r["format_bytes"],
-                         "GBs_reference_formula": round(r["ref_gbs"], 1) if r["ref_gbs"] else None}
-            for r in results}
+    if yref is not None and out["kernels"]:
+        best = min(out["kernels"].items(), key=lambda kv: kv[1]["us"])
+        out["gpu_over_cpu"] = round(out["cpu"]["ms"] * 1e3 / best[1]["us"], 1)
+    return out
 
 
 def reference_csr_bench():
@@ -230,7 +342,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bound on the CPU baseline's timed work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--formats", default="all", help="comma list of kernel labels to measure, or 'all'")
-    ap.add_argument("--sections", default="lognormal,lognormal_fp32,stencil27,spmm,copy,serving,batched")
+    ap.add_argument("--sections",
+                    default="lognormal,lognormal_fp32,stencil27,stencil27_fp32,laplace2d,spmm,copy,serving,batched")
     ap.add_argument("--serve-matrices", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--serve-rates", default="5000,20000,50000")
@@ -250,18 +363,27 @@ def main():
     sections = set(args.sections.split(","))
     wanted = None if args.formats == "all" else set(args.formats.split(","))
 
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.time()
     A = lib.generate(lib.GEN_LOGNORMAL, LOGNORMAL, 2, np.float64)
     x = lib.fill_uniform(A.n, 4, np.float64)
     log(f"generated lognormal M={A.m} nnz={A.nnz} in {time.time() - t0:.1f}s")
+    # CPU baseline first: its y (the reference's FP64 kernel) is the reference
+    # every GPU kernel's error is stated against, FP32 ones included
+    y_cpu, cpu = None, None
+    if not args.no_cpu:
+        ms, iters, y_cpu, kind, what = cpu_baseline(A, x, threads, args.cpu_seconds)
+        cpu = (ms, iters, kind, what)
     M = lib.Matrix.from_host(A)
     dx = lib.DeviceArray.from_numpy(x)
     dy = lib.DeviceArray.empty(A.m, np.float64)
-    results = measure(lib, M, dx, dy, "lognormal", wanted, args.warmup)
+    results, refused = measure(lib, M, dx, dy, "lognormal", wanted, args.warmup, y_cpu)
     best = min(results, key=lambda r: r["ms"])  # fastest kernel on this matrix
     fmt, variant = best["fmt"], best["variant"]
     if fmt != lib.CSR:
         M.convert(fmt, best["params"])
+    else:
+        M.prepare(fmt, variant)  # the layout is built before the timed region
 
     # timed region: K steps bracketed by device syncs; the kernel's own device
     # time comes from hipEvents recorded on the stream it runs on
@@ -302,38 +424,45 @@ def main():
             "algorithmic_bytes": eff,
             "kernel_ms": round(dev_ms, 5),
         },
-        "formats": {"lognormal_f64": summary(results)},
+        "formats": {"lognormal_f64": summary(results, refused)},
     }
-    if "copy" in sections:
-        gbs, _ = copy_bandwidth(lib)
-        out["copy_GBs"] = round(gbs, 1)
-    if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ms, iters, y_cpu, kind, what = cpu_baseline(A, x, threads, args.cpu_seconds)
+    if cpu is not None:
+        ms, iters, kind, what = cpu
         out["cpu_baseline"] = {
             "value": round(M.effective_bytes(lib.CSR) / (ms * 1e-3) / 1e9, 3),
             "unit": "GB/s",
             "cores": threads,
             "kind": kind,
             "sample": f"the full lognormal matrix, {iters} CSR SpMVs ({what}, OpenMP), {ms:.2f} ms each",
+            "host": host_info(),
         }
-        nb = np.linalg.norm(y_cpu)
-        out["parity_rel2_vs_cpu"] = float(np.linalg.norm(y_gpu - y_cpu) / (nb if nb else 1.0))
+        out["parity_rel2_vs_cpu"] = rel2(y_gpu, y_cpu)
     del M, dx, dy
+    if "copy" in sections:
+        gbs, _ = copy_bandwidth(lib)
+        out["copy_GBs"] = round(gbs, 1)
     if "lognormal_fp32" in sections:
         A32 = lib.generate(lib.GEN_LOGNORMAL, LOGNORMAL, 2, np.float32)
         M = lib.Matrix.from_host(A32)
         dx = lib.DeviceArray.from_numpy(x.astype(np.float32))
         dy = lib.DeviceArray.empty(A32.m, np.float32)
-        out["formats"]["lognormal_f32"] = summary(measure(lib, M, dx, dy, "lognormal_fp32", wanted, args.warmup))
+        out["formats"]["lognormal_f32"] = summary(*measure(lib, M, dx, dy, "lognormal_fp32", wanted, args.warmup,
+                                                           y_cpu))
         del M, dx, dy, A32
-    if "stencil27" in sections:
-        S = lib.generate(lib.GEN_STENCIL27, [160, 26.0, -1.0], 0, np.float64)
+    del A, y_cpu
+    for dt, key in ((np.float64, "stencil27"), (np.float32, "stencil27_fp32")):
+        if key not in sections:
+            continue
+        S = lib.generate(lib.GEN_STENCIL27, [160, 26.0, -1.0], 0, dt)
         M = lib.Matrix.from_host(S)
-        dx = lib.DeviceArray.from_numpy(lib.fill_uniform(S.n, 5))
-        dy = lib.DeviceArray.empty(S.m, np.float64)
-        out["formats"]["stencil27_f64"] = summary(measure(lib, M, dx, dy, "stencil27", wanted, args.warmup))
+        dx = lib.DeviceArray.from_numpy(lib.fill_uniform(S.n, 5, dt))
+        dy = lib.DeviceArray.empty(S.m, dt)
+        name = "stencil27_f64" if dt == np.float64 else "stencil27_f32"
+        out["formats"][name] = summary(*measure(lib, M, dx, dy, key, wanted, args.warmup))
         del M, dx, dy, S
+    if "laplace2d" in sections:
+        out["laplace2d_1024"] = laplace_section(lib, max(args.steps, 50), args.warmup, not args.no_cpu,
+                                                args.cpu_seconds, threads)
     if "spmm" in sections:
         out["spmm_blockrand4x4_1M"] = spmm_section(lib, 20)
     if "serving" in sections:

@@ -176,6 +176,14 @@ int lespmv_spmm(lespmv_matrix_t A, int fmt, int variant, int32_t k, double alpha
  * measured with hipEvents on the stream the kernels run on. Synchronous. */
 int lespmv_time_spmv(lespmv_matrix_t A, int fmt, int variant, double alpha, const void* x, double beta, void* y,
                      int32_t iters, double* ms_per_call);
+/* Average device ms per call of `iters` rounds over `count` handles of one
+ * format, each round one SpMV per handle (handles[i] with xs[i], ys[i]) in
+ * turn, all on handles[0]'s stream: matrices that together exceed the 256 MB
+ * last-level cache are then read from HBM on every call, as a single small
+ * matrix repeated would not be. Synchronous (waits for the device first). */
+int lespmv_time_spmv_rotating(const lespmv_matrix_t* handles, int32_t count, int fmt, int variant, double alpha,
+                              const void* const* xs, double beta, void* const* ys, int32_t iters,
+                              double* ms_per_call);
 int lespmv_time_spmm(lespmv_matrix_t A, int fmt, int variant, int32_t k, double alpha, const void* X, int64_t ldx,
                      double beta, void* Y, int64_t ldy, int32_t iters, double* ms_per_call);
 

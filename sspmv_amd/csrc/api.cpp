@@ -62,9 +62,8 @@ struct StreamOrder {
   }
 };
 
-void run_spmv(Matrix& A, int fmt, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
-  require_format(A, fmt);
-  StreamOrder order(A, s);
+void dispatch_spmv(Matrix& A, int fmt, int variant, double alpha, const void* x, double beta, void* y,
+                   hipStream_t s) {
   switch (fmt) {
     case LESPMV_CSR: csr_spmv(A, variant, alpha, x, beta, y, s); break;
     case LESPMV_COO: coo_spmv(A, variant, alpha, x, beta, y, s); break;
@@ -77,6 +76,12 @@ void run_spmv(Matrix& A, int fmt, int variant, double alpha, const void* x, doub
     case LESPMV_CSR5: csr5_spmv(A, alpha, x, beta, y, s); break;
     default: throw Error(LESPMV_ERR_ARG, "bad format");
   }
+}
+
+void run_spmv(Matrix& A, int fmt, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  require_format(A, fmt);
+  StreamOrder order(A, s);
+  dispatch_spmv(A, fmt, variant, alpha, x, beta, y, s);
   order.done();
 }
 
@@ -453,6 +458,34 @@ int lespmv_time_spmv(lespmv_matrix_t h, int fmt, int variant, double alpha, cons
     float ms = 0;
     LSP_HIP(hipEventElapsedTime(&ms, A.ev0, A.ev1));
     *ms_per_call = ms / iters;
+  });
+}
+
+int lespmv_time_spmv_rotating(const lespmv_matrix_t* handles, int32_t count, int fmt, int variant, double alpha,
+                              const void* const* xs, double beta, void* const* ys, int32_t iters,
+                              double* ms_per_call) {
+  return guarded([&] {
+    LSP_REQUIRE(handles && xs && ys && count > 0 && iters > 0 && ms_per_call, LESPMV_ERR_ARG,
+                "bad rotating-timing arguments");
+    check_fmt(fmt);
+    std::vector<Matrix*> mats(count);
+    for (int i = 0; i < count; i++) {
+      mats[i] = &mat(handles[i]);
+      require_format(*mats[i], fmt);
+      if (fmt == LESPMV_CSR) csr_prepare(*mats[i], variant, mats[i]->stream);
+    }
+    LSP_HIP(hipDeviceSynchronize());  // every handle's earlier calls are done
+    Matrix& A0 = *mats[0];
+    hipStream_t s = A0.stream;
+    LSP_HIP(hipEventRecord(A0.ev0, s));
+    for (int it = 0; it < iters; it++)
+      for (int i = 0; i < count; i++) dispatch_spmv(*mats[i], fmt, variant, alpha, xs[i], beta, ys[i], s);
+    LSP_HIP(hipEventRecord(A0.ev1, s));
+    LSP_HIP(hipEventSynchronize(A0.ev1));
+    float ms = 0;
+    LSP_HIP(hipEventElapsedTime(&ms, A0.ev0, A0.ev1));
+    for (Matrix* M : mats) M->last_stream = nullptr;  // every call above has completed
+    *ms_per_call = ms / ((double)iters * count);
   });
 }
 

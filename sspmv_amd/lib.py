@@ -133,6 +133,8 @@ def load() -> C.CDLL:
         "lespmv_time_spmv": ([vp, C.c_int, C.c_int, dbl, vp, dbl, vp, i32, C.POINTER(dbl)], C.c_int),
         "lespmv_time_spmm": ([vp, C.c_int, C.c_int, i32, dbl, vp, i64, dbl, vp, i64, i32, C.POINTER(dbl)],
                              C.c_int),
+        "lespmv_time_spmv_rotating": ([C.POINTER(vp), i32, C.c_int, C.c_int, dbl, C.POINTER(vp), dbl,
+                                       C.POINTER(vp), i32, C.POINTER(dbl)], C.c_int),
         "lespmv_effective_bytes": ([vp, C.c_int, i32, C.c_int, C.POINTER(i64)], C.c_int),
         "lespmv_reference_bytes": ([vp, C.c_int, C.POINTER(i64)], C.c_int),
         "lespmv_format_bytes": ([vp, C.c_int, C.POINTER(i64)], C.c_int),
@@ -652,6 +654,18 @@ def features(M: "Matrix", symmetric: bool = False, images: bool = False):
     check(lib.lespmv_features_vector(C.byref(f), vec.ctypes.data_as(C.POINTER(C.c_double))))
     vals = {n: getattr(f, n) for n, _ in FeaturesC._fields_}
     return vals, vec, img
+
+
+def time_spmv_rotating(mats, fmt: int, alpha: float, xs, beta: float, ys, iters: int, variant: int = 0) -> float:
+    """Device ms per SpMV over `iters` rounds of one call per matrix in turn
+    (each matrix with its own x and y), on the first matrix's stream."""
+    n = len(mats)
+    H = (C.c_void_p * n)(*[M.h.value for M in mats])
+    X = (C.c_void_p * n)(*[x.ptr.value for x in xs])
+    Y = (C.c_void_p * n)(*[y.ptr.value for y in ys])
+    ms = C.c_double()
+    check(load().lespmv_time_spmv_rotating(H, n, fmt, variant, alpha, X, beta, Y, iters, C.byref(ms)))
+    return ms.value
 
 
 def time_copy(dst: DeviceArray, src: DeviceArray, nbytes: int, iters: int) -> float:
