@@ -5,7 +5,9 @@
 // they lie (see build_ref.sh); nothing of the reference is copied. It calls the
 // reference's conversions (include/sparse_conversion.h) and LeSpMV_<fmt> entry
 // points (src/spmv_*.cpp) and returns their arrays in the same named layout as
-// the oracle (oracle/lespmv_oracle.cpp), so the two can be compared directly.
+// the oracle (oracle/lespmv_oracle.cpp), so the two can be compared directly,
+// with the reference's own bytes_per_spmv (include/spmv_benchmark.h:26-150)
+// of each converted struct as the scalar "bytes_per_spmv".
 #include "LeSpMV.h"
 
 #include <set>
@@ -59,6 +61,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
   int rc = 0;
   switch (fmt) {
     case 1: {
+      r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(csr));
       csr.kernel_flag = flag;
       if (flag == 2) set_partition_csr(csr);
       if (y) LeSpMV_csr(alpha, csr, x, beta, y);
@@ -69,6 +72,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
       r.put("row_index", coo.row_index, nnz);
       r.put("col_index", coo.col_index, nnz);
       r.put("values", coo.values, nnz);
+      r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(coo));
       coo.kernel_flag = flag;
       if (y) LeSpMV_coo(alpha, coo, x, beta, y);
       delete_coo_matrix(coo);
@@ -85,6 +89,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
       r.put("diag_offsets", dia.diag_offsets, dia.complete_ndiags);
       r.put("diag_data", dia.diag_data, (size_t)dia.complete_ndiags * dia.stride);
       r.scalar("stride", dia.stride);
+      r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(dia));
       dia.kernel_flag = flag;
       if (y) LeSpMV_dia(alpha, dia, x, beta, y);
       delete_dia_matrix(dia);
@@ -98,6 +103,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
       r.scalar("ld", (int)ell.ld);
       ell.kernel_flag = flag;
       ell.num_nnzs = nnz;
+      r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(ell));
       if (y) LeSpMV_ell(alpha, ell, x, beta, y);
       delete_ell_matrix(ell);
       break;
@@ -109,6 +115,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
       r.scalar("chunk_num", s.chunk_num);
       r.scalar("sliceWidth", s.sliceWidth);
       r.scalar("alignment", s.alignment);
+      r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(s));
       s.kernel_flag = flag;
       if (y) LeSpMV_sell(alpha, s, x, beta, y);
       delete_s_ell_matrix(s);
@@ -127,6 +134,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
       r.scalar("chunkNum", s.chunkNum);
       r.scalar("validchunkNum", s.validchunkNum);
       r.scalar("alignment", s.alignment);
+      r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(s));
       s.kernel_flag = flag;
       if (y) LeSpMV_sell_c_sigma(alpha, s, x, beta, y);
       delete_s_ell_c_sigma_matrix(s);
@@ -140,6 +148,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
       r.scalar("chunkWidth_C", s.chunkWidth_C);
       r.scalar("validchunkNum", s.validchunkNum);
       r.scalar("alignment", s.alignment);
+      r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(s));
       s.kernel_flag = flag;
       if (y) LeSpMV_sell_c_R(alpha, s, x, beta, y);
       delete_s_ell_c_R_matrix(s);
@@ -163,6 +172,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
         r.scalar("num_offsets", c.num_offsets);
         r.scalar("p", c._p);
         r.scalar("tail_partition_start", c.tail_partition_start);
+        r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(c));
         if (y) LeSpMV_csr5(alpha, c, x, beta, y);
         delete_csr5_matrix(c);
       } else {
@@ -180,6 +190,7 @@ int run(NamedArrays& r, int fmt, int m, int n, int nnz, const int* rp, const int
       r.scalar("nnzb", b.nnzb);
       r.scalar("blockDim_r", b.blockDim_r);
       r.scalar("blockDim_c", b.blockDim_c);
+      r.scalar("bytes_per_spmv", (int64_t)bytes_per_spmv(b));
       b.kernel_flag = flag;
       if (flag == 2) {
         const int t = Le_get_thread_num();

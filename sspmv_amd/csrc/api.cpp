@@ -100,6 +100,7 @@ void run_spmm(Matrix& A, int fmt, int variant, int k, double alpha, const void* 
     case LESPMV_SELL_C_R: sliced_spmm(A, fmt, k, alpha, X, ldx, beta, Y, ldy, s); break;
     case LESPMV_DIA: dia_spmm(A, variant, k, alpha, X, ldx, beta, Y, ldy, s); break;
     case LESPMV_BSR: bsr_spmm(A, variant, k, alpha, X, ldx, beta, Y, ldy, s); break;
+    case LESPMV_CSR5: csr5_spmm(A, k, alpha, X, ldx, beta, Y, ldy, s); break;
     default: throw Error(LESPMV_ERR_UNSUPPORTED, "SpMM is not provided for this format");
   }
   order.done();
@@ -117,9 +118,12 @@ int64_t effective_bytes(Matrix& A, int fmt, int k, bool beta_nz) {
     case LESPMV_SELL:
     case LESPMV_SELL_C_SIGMA:
     case LESPMV_SELL_C_R: {
+      // ELL is described by one width (max_row_width); the sliced formats by one
+      // length per chunk and, when sorted, the row order
       auto* d = static_cast<SlicedData*>(A.fmt[fmt].get());
       const bool sorted = d->reorder.p != nullptr;
-      return A.nnz * (sv + si) + d->nchunks * si + (sorted ? A.m * si : 0) + vec;
+      const int64_t chunks = fmt == LESPMV_ELL ? 0 : d->nchunks;
+      return A.nnz * (sv + si) + chunks * si + (sorted ? A.m * si : 0) + vec;
     }
     case LESPMV_DIA: {
       auto* d = static_cast<DiaData*>(A.fmt[fmt].get());

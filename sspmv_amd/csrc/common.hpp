@@ -242,6 +242,8 @@ void bsr_spmm(Matrix& A, int variant, int k, double alpha, const void* X, int64_
 
 void convert_csr5(Matrix& A, const lespmv_params_t& p, hipStream_t s);
 void csr5_spmv(Matrix& A, double alpha, const void* x, double beta, void* y, hipStream_t s);
+void csr5_spmm(Matrix& A, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
+               hipStream_t s);
 void csr5_export(Matrix& A, const std::string& name, std::vector<unsigned char>& out);
 
 // Generic device primitives (primitives.hip).

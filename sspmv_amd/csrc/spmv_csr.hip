@@ -365,15 +365,29 @@ __global__ void panel_reduce_vec(int64_t m, int P, const V* __restrict__ partial
 }
 
 // row-end bits of the panels' merge paths: row i of panel p ends at item
-// i + rp_p[i + 1]
-__global__ void panel_ends(int64_t m, int P, const int* __restrict__ prp, const int64_t* __restrict__ endoff,
-                           uint32_t* __restrict__ ends) {
+// i + rp_p[i + 1]. Consecutive threads take consecutive rows, so the bits of
+// one word come from a run of neighbouring lanes: the run is OR-combined with
+// shuffles and its first lane sets the word with one atomic.
+__global__ __launch_bounds__(kBlock) void panel_ends(int64_t m, int P, const int* __restrict__ prp,
+                                                     const int64_t* __restrict__ endoff, uint32_t* __restrict__ ends) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (int64_t)P * m) return;
-  const int p = (int)(g / m);
-  const int64_t i = g - (int64_t)p * m;
-  const int64_t k = i + prp[(int64_t)p * (m + 1) + i + 1];
-  atomicOr(ends + endoff[p] + (k >> 5), 1u << (k & 31));
+  const int lane = threadIdx.x & 63;
+  int64_t word = -1;  // global word index; runs of equal words are contiguous along the lanes
+  uint32_t bits = 0;
+  if (g < (int64_t)P * m) {
+    const int p = (int)(g / m);
+    const int64_t i = g - (int64_t)p * m;
+    const int64_t k = i + prp[(int64_t)p * (m + 1) + i + 1];
+    word = endoff[p] + (k >> 5);
+    bits = 1u << (k & 31);
+  }
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t b = __shfl_down(bits, off);
+    const long long w = __shfl_down((long long)word, off);
+    if (lane + off < 64 && w == word) bits |= b;
+  }
+  const long long prev = __shfl_up((long long)word, 1);
+  if (word >= 0 && (lane == 0 || prev != word)) atomicOr(ends + word, bits);
 }
 
 // panel-local columns of entries 2t and 2t + 1: low 16 bits, and the high
@@ -395,12 +409,42 @@ __global__ void panel_compact(int64_t nnz, int P, int64_t width, const int64_t* 
   hi[t] = (uint8_t)byte;
 }
 
-// per (panel, row) entry counts, each row's counts by its own thread
-__global__ void panel_count(int64_t m, int P, int64_t width, const int* __restrict__ rp, const int* __restrict__ ci,
-                            int64_t* __restrict__ cnt) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
-  for (int j = rp[r]; j < rp[r + 1]; j++) cnt[(int64_t)(ci[j] / width) * (m + 1) + r]++;
+// Panel build: a wave takes 64 consecutive rows, whose entries are
+// contiguous in the CSR arrays, and walks them 64 at a time (coalesced).
+constexpr int kMaxPanels = 16;
+
+// Row (0..63) of the wave's rows holding entry `idx`: the number of rows whose
+// end is <= idx; lane l holds the end of row l (INT32_MAX past the last row).
+__device__ __forceinline__ int row_in_wave(int rend, int idx) {
+  int lo = 0;
+  for (int step = 32; step > 0; step >>= 1)
+    if (__shfl(rend, lo + step - 1) <= idx) lo += step;
+  return lo;
+}
+
+// per (panel, row) entry counts: LDS counters for the wave's 64 rows x P
+// panels, written out panel by panel
+__global__ __launch_bounds__(kBlock) void panel_count(int64_t m, int P, int64_t width, const int* __restrict__ rp,
+                                                      const int* __restrict__ ci, int64_t* __restrict__ cnt) {
+  __shared__ int counts[kBlock / 64][64 * kMaxPanels];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int* c = counts[w];
+  for (int i = lane; i < 64 * kMaxPanels; i += 64) c[i] = 0;
+  __syncthreads();
+  const int64_t r0 = ((int64_t)blockIdx.x * (kBlock / 64) + w) * 64;
+  const int nr = r0 < m ? (int)(m - r0 < 64 ? m - r0 : 64) : 0;
+  if (nr > 0) {
+    const int rend = lane < nr ? rp[r0 + lane + 1] : INT32_MAX;
+    const int s = rp[r0], e = rp[r0 + nr];
+    for (int e0 = s; e0 < e; e0 += 64) {
+      const int idx = e0 + lane;
+      const int row = row_in_wave(rend, idx);
+      if (idx < e) atomicAdd(&c[row * P + (int)(ci[idx] / width)], 1);
+    }
+  }
+  __syncthreads();
+  if (lane < nr)
+    for (int q = 0; q < P; q++) cnt[(int64_t)q * (m + 1) + r0 + lane] = c[lane * P + q];
 }
 
 __global__ void panel_rp(int64_t n, const int64_t* __restrict__ in, int* __restrict__ out) {
@@ -408,20 +452,40 @@ __global__ void panel_rp(int64_t n, const int64_t* __restrict__ in, int* __restr
   if (i < n) out[i] = (int)in[i];
 }
 
+// Entries to their panels, order kept: panel q's entries of the wave's rows go
+// to consecutive slots from nzoff[q] + prp_q[r0] (held by lane q); an entry's
+// slot adds the number of earlier lanes of its step bound for the same panel.
 template <class V>
-__global__ void panel_scatter(int64_t m, int P, int64_t width, const int* __restrict__ rp, const int* __restrict__ ci,
-                              const V* __restrict__ val, const int64_t* __restrict__ nzoff,
-                              const int* __restrict__ prp, int* __restrict__ pci, V* __restrict__ pval) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
-  int fill[16];
-  for (int p = 0; p < P; p++) fill[p] = 0;
-  for (int j = rp[r]; j < rp[r + 1]; j++) {
-    const int c = ci[j];
-    const int p = (int)(c / width);
-    const int64_t d = nzoff[p] + prp[(int64_t)p * (m + 1) + r] + fill[p]++;
-    pci[d] = c;
-    pval[d] = val[j];
+__global__ __launch_bounds__(kBlock) void panel_scatter(int64_t m, int P, int64_t width, const int* __restrict__ rp,
+                                                        const int* __restrict__ ci, const V* __restrict__ val,
+                                                        const int64_t* __restrict__ nzoff,
+                                                        const int* __restrict__ prp, int* __restrict__ pci,
+                                                        V* __restrict__ pval) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r0 = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 64;
+  if (r0 >= m) return;  // whole waves leave together
+  const int nr = (int)(m - r0 < 64 ? m - r0 : 64);
+  long long next = lane < P ? nzoff[lane] + prp[(int64_t)lane * (m + 1) + r0] : 0;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int s = rp[r0], e = rp[r0 + nr];
+  for (int e0 = s; e0 < e; e0 += 64) {
+    const int idx = e0 + lane;
+    const bool ok = idx < e;
+    const int c = ok ? ci[idx] : 0;
+    const int p = ok ? (int)(c / width) : -1;
+    long long dst = 0;
+    int taken = 0;
+    for (int q = 0; q < P; q++) {
+      const uint64_t mk = __ballot(p == q);
+      const long long base = __shfl(next, q);
+      if (p == q) dst = base + __popcll(mk & below);
+      if (lane == q) taken = __popcll(mk);
+    }
+    next += taken;
+    if (ok) {
+      pci[dst] = c;
+      pval[dst] = val[idx];
+    }
   }
 }
 
@@ -517,6 +581,7 @@ void build_panels(Matrix& A, int P, hipStream_t s) {
   DBuf cnt;
   cnt.alloc(sizeof(int64_t) * P * (m + 1));
   LSP_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes, s));
+  LSP_REQUIRE(P <= kMaxPanels, LESPMV_ERR_UNSUPPORTED, "at most 16 column panels");
   hipLaunchKernelGGL(panel_count, grid1d(m, kBlock), dim3(kBlock), 0, s, m, P, width, A.rp.as<int>(), A.ci.as<int>(),
                      cnt.as<int64_t>());
   LSP_LAUNCHED();

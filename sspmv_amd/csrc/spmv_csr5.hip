@@ -189,22 +189,35 @@ __global__ void compact_list(int64_t lim, const int* __restrict__ flag, const in
   if (i < lim && flag[i]) list[pos[i]] = (int)i;
 }
 
-template <class V>
-__global__ void scale_list(int64_t n, const int* __restrict__ list, V* __restrict__ y, V beta) {
+// Element i of a vector: x[i] for SpMV (S false), X[i * ld] for one column of
+// a row-major multi-vector (S true: SpMM runs the kernels once per column).
+template <bool S>
+__device__ __forceinline__ int64_t at(int64_t i, int64_t ld) {
+  return S ? i * ld : i;
+}
+
+template <class V, bool S>
+__global__ void scale_list(int64_t n, const int* __restrict__ list, V* __restrict__ y, V beta, int64_t ldy) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[list[i]] = beta == V(0) ? V(0) : beta * y[list[i]];
+  if (i < n) {
+    V* yr = y + at<S>(list[i], ldy);
+    *yr = beta == V(0) ? V(0) : beta * *yr;
+  }
 }
 
 // ---- SpMV -------------------------------------------------------------------
 
-template <class V, int W>
+template <class V, int W, bool S>
 __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int bit_y, int bit_all,
                                                      const uint32_t* __restrict__ tp,
                                                      const uint32_t* __restrict__ desc,
                                                      const int* __restrict__ dop, const int* __restrict__ dof,
                                                      const int* __restrict__ col, const V* __restrict__ val,
                                                      const V* __restrict__ x, V* __restrict__ y,
-                                                     V* __restrict__ calib, V alpha, V beta) {
+                                                     V* __restrict__ calib, V alpha, V beta, int64_t ldx,
+                                                     int64_t ldy) {
+  // y row r of this column
+  auto Y = [&](int64_t r) -> V& { return y[at<S>(r, ldy)]; };
   const int l = threadIdx.x & (W - 1);
   const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / W;
   if (t >= p - 1) return;  // whole W-lane groups leave together
@@ -227,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
   }
 #pragma unroll
   for (int i = 0; i < 32; i++)
-    if (i < sigma) v[i] *= x[c[i]];
+    if (i < sigma) v[i] *= x[at<S>(c[i], ldx)];
   if (rs == re) {  // the whole tile lies inside one row
     V s = 0;
 #pragma unroll
@@ -237,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
     for (int o = W / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, W);
     if (l == 0) {
       if (first_direct) {
-        y[rs] = beta == V(0) ? alpha * s : alpha * s + beta * y[rs];
+        Y(rs) = beta == V(0) ? alpha * s : alpha * s + beta * Y(rs);
         calib[t] = 0;
       } else {
         calib[t] = s;
@@ -264,12 +277,12 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
         head = acc;
         seg = (l == 0) ? 0 : (int)y_off + 1;  // lane 0 opens segment 0 at position 0
       } else if (seg == 0) {
-        if (first_direct) y[rs] = beta == V(0) ? alpha * acc : alpha * acc + beta * y[rs];
+        if (first_direct) Y(rs) = beta == V(0) ? alpha * acc : alpha * acc + beta * Y(rs);
         else calib[t] = acc;
         seg = 1;
       } else {
         const int64_t r = row_of(seg);
-        y[r] = beta == V(0) ? alpha * acc : alpha * acc + beta * y[r];
+        Y(r) = beta == V(0) ? alpha * acc : alpha * acc + beta * Y(r);
         seg++;
       }
       acc = 0;
@@ -298,21 +311,21 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
   if (!has_flag) return;
   const V total = acc + (l + 1 < W ? carry_all : V(0));
   if (seg == 0) {
-    if (first_direct) y[rs] = beta == V(0) ? alpha * total : alpha * total + beta * y[rs];
+    if (first_direct) Y(rs) = beta == V(0) ? alpha * total : alpha * total + beta * Y(rs);
     else calib[t] = total;
   } else {
     const int64_t r = row_of(seg);
-    y[r] = beta == V(0) ? alpha * total : alpha * total + beta * y[r];
+    Y(r) = beta == V(0) ? alpha * total : alpha * total + beta * Y(r);
   }
   if (l == 0 && first_direct) calib[t] = 0;
 }
 
 // Continuation rows: the first continuation tile of a chain adds the
 // calibrators of the chain in tile order (reference calibrate kernel, :367-384).
-template <class V>
+template <class V, bool S>
 __global__ void csr5_calibrate(int64_t p, int W, int bit_all, const uint32_t* __restrict__ tp,
                                const uint32_t* __restrict__ desc, const V* __restrict__ calib, V* __restrict__ y,
-                               V alpha) {
+                               V alpha, int64_t ldy) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= p - 1) return;
   auto direct = [&](int64_t q) { return (desc[q * W] >> (31 - bit_all)) & 1; };
@@ -321,59 +334,64 @@ __global__ void csr5_calibrate(int64_t p, int W, int bit_all, const uint32_t* __
   if (t > 0 && (tp[t - 1] & kMask) == r && !direct(t - 1)) return;  // not the chain head
   V sum = 0;
   for (int64_t q = t; q < p - 1 && (tp[q] & kMask) == r && !direct(q); q++) sum += calib[q];
-  y[r] += alpha * sum;
+  y[at<S>(r, ldy)] += alpha * sum;
 }
 
 // Rows after the last full tile (reference tail kernel, :386-424).
-template <class V>
+template <class V, bool S>
 __global__ void csr5_tail(int64_t m, int64_t tail_start, int64_t first_tail, const int* __restrict__ rp,
                           const int* __restrict__ col, const V* __restrict__ val, const V* __restrict__ x,
-                          V* __restrict__ y, V alpha, V beta) {
+                          V* __restrict__ y, V alpha, V beta, int64_t ldx, int64_t ldy) {
   const int64_t row = tail_start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= m) return;
   const int64_t s = row == tail_start ? first_tail : rp[row];
   V sum = 0;
-  for (int64_t i = s; i < rp[row + 1]; i++) sum += val[i] * x[col[i]];
-  if (row == tail_start && rp[row] != first_tail) y[row] += alpha * sum;
-  else y[row] = beta == V(0) ? alpha * sum : alpha * sum + beta * y[row];
+  for (int64_t i = s; i < rp[row + 1]; i++) sum += val[i] * x[at<S>(col[i], ldx)];
+  V* yr = y + at<S>(row, ldy);
+  if (row == tail_start && rp[row] != first_tail) *yr += alpha * sum;
+  else *yr = beta == V(0) ? alpha * sum : alpha * sum + beta * *yr;
 }
 
-template <class V, int W>
-void launch_tiles(Matrix& A, Csr5Data& d, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+template <class V, int W, bool S>
+void launch_tiles(Matrix& A, Csr5Data& d, double alpha, const void* x, double beta, void* y, int64_t ldx,
+                  int64_t ldy, hipStream_t s) {
   const int bit_all = d.bit_y + d.bit_scan;
-  hipLaunchKernelGGL((csr5_tiles<V, W>), grid1d((d.p - 1) * W, kBlock), dim3(kBlock), 0, s, d.p, d.sigma, d.bit_y,
-                     bit_all, d.tile_ptr.as<uint32_t>(), d.tile_desc.as<uint32_t>(), d.desc_off_ptr.as<int>(),
-                     d.desc_off.as<int>(), d.col.as<int>(), d.val.as<V>(), static_cast<const V*>(x),
-                     static_cast<V*>(y), d.calib.as<V>(), (V)alpha, (V)beta);
+  hipLaunchKernelGGL((csr5_tiles<V, W, S>), grid1d((d.p - 1) * W, kBlock), dim3(kBlock), 0, s, d.p, d.sigma,
+                     d.bit_y, bit_all, d.tile_ptr.as<uint32_t>(), d.tile_desc.as<uint32_t>(),
+                     d.desc_off_ptr.as<int>(), d.desc_off.as<int>(), d.col.as<int>(), d.val.as<V>(),
+                     static_cast<const V*>(x), static_cast<V*>(y), d.calib.as<V>(), (V)alpha, (V)beta, ldx, ldy);
   LSP_LAUNCHED();
 }
 
-template <class V>
-void run(Matrix& A, Csr5Data& d, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+// One product: y = alpha A x + beta y, with x and y strided by ldx / ldy
+// when S (one column of a row-major multi-vector).
+template <class V, bool S>
+void run(Matrix& A, Csr5Data& d, double alpha, const void* x, double beta, void* y, int64_t ldx, int64_t ldy,
+         hipStream_t s) {
   V* yv = static_cast<V*>(y);
   if (d.num_empty) {
-    hipLaunchKernelGGL((scale_list<V>), grid1d(d.num_empty, kBlock), dim3(kBlock), 0, s, d.num_empty,
-                       d.empty_rows.as<int>(), yv, (V)beta);
+    hipLaunchKernelGGL((scale_list<V, S>), grid1d(d.num_empty, kBlock), dim3(kBlock), 0, s, d.num_empty,
+                       d.empty_rows.as<int>(), yv, (V)beta, ldy);
     LSP_LAUNCHED();
   }
   if (d.p > 1) {
     switch (d.omega) {
-      case 64: launch_tiles<V, 64>(A, d, alpha, x, beta, y, s); break;
-      case 32: launch_tiles<V, 32>(A, d, alpha, x, beta, y, s); break;
-      case 16: launch_tiles<V, 16>(A, d, alpha, x, beta, y, s); break;
-      case 8: launch_tiles<V, 8>(A, d, alpha, x, beta, y, s); break;
-      case 4: launch_tiles<V, 4>(A, d, alpha, x, beta, y, s); break;
+      case 64: launch_tiles<V, 64, S>(A, d, alpha, x, beta, y, ldx, ldy, s); break;
+      case 32: launch_tiles<V, 32, S>(A, d, alpha, x, beta, y, ldx, ldy, s); break;
+      case 16: launch_tiles<V, 16, S>(A, d, alpha, x, beta, y, ldx, ldy, s); break;
+      case 8: launch_tiles<V, 8, S>(A, d, alpha, x, beta, y, ldx, ldy, s); break;
+      case 4: launch_tiles<V, 4, S>(A, d, alpha, x, beta, y, ldx, ldy, s); break;
       default: throw Error(LESPMV_ERR_UNSUPPORTED, "CSR5 omega must be a power of two <= 64");
     }
-    hipLaunchKernelGGL((csr5_calibrate<V>), grid1d(d.p - 1, kBlock), dim3(kBlock), 0, s, d.p, d.omega,
+    hipLaunchKernelGGL((csr5_calibrate<V, S>), grid1d(d.p - 1, kBlock), dim3(kBlock), 0, s, d.p, d.omega,
                        d.bit_y + d.bit_scan, d.tile_ptr.as<uint32_t>(), d.tile_desc.as<uint32_t>(), d.calib.as<V>(),
-                       yv, (V)alpha);
+                       yv, (V)alpha, ldy);
     LSP_LAUNCHED();
   }
   const int64_t first_tail = (d.p - 1) * (int64_t)d.omega * d.sigma;
-  hipLaunchKernelGGL((csr5_tail<V>), grid1d(A.m - d.tail_start, kBlock), dim3(kBlock), 0, s, A.m, d.tail_start,
-                     first_tail, A.rp.as<int>(), d.col.as<int>(), d.val.as<V>(), static_cast<const V*>(x), yv,
-                     (V)alpha, (V)beta);
+  hipLaunchKernelGGL((csr5_tail<V, S>), grid1d(A.m - d.tail_start, kBlock), dim3(kBlock), 0, s, A.m,
+                     d.tail_start, first_tail, A.rp.as<int>(), d.col.as<int>(), d.val.as<V>(),
+                     static_cast<const V*>(x), yv, (V)alpha, (V)beta, ldx, ldy);
   LSP_LAUNCHED();
 }
 
@@ -483,8 +501,25 @@ void convert_csr5(Matrix& A, const lespmv_params_t& prm, hipStream_t s) {
 void csr5_spmv(Matrix& A, double alpha, const void* x, double beta, void* y, hipStream_t s) {
   auto& d = *static_cast<Csr5Data*>(A.fmt[LESPMV_CSR5].get());
   if (A.m == 0) return;
-  if (A.dtype == LESPMV_F64) run<double>(A, d, alpha, x, beta, y, s);
-  else run<float>(A, d, alpha, x, beta, y, s);
+  if (A.dtype == LESPMV_F64) run<double, false>(A, d, alpha, x, beta, y, 1, 1, s);
+  else run<float, false>(A, d, alpha, x, beta, y, 1, 1, s);
+}
+
+// SpMM column by column: the tile, calibration and tail kernels once per
+// right-hand side, reading column j of X and writing column j of Y in place
+// (the per-tile calibrators are reused, in stream order). Each column is
+// therefore the SpMV of that column, bit for bit.
+void csr5_spmm(Matrix& A, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
+               hipStream_t s) {
+  auto& d = *static_cast<Csr5Data*>(A.fmt[LESPMV_CSR5].get());
+  if (A.m == 0) return;
+  const size_t sv = vsize(A.dtype);
+  for (int j = 0; j < k; j++) {
+    const void* xj = static_cast<const char*>(X) + j * sv;
+    void* yj = static_cast<char*>(Y) + j * sv;
+    if (A.dtype == LESPMV_F64) run<double, true>(A, d, alpha, xj, beta, yj, ldx, ldy, s);
+    else run<float, true>(A, d, alpha, xj, beta, yj, ldx, ldy, s);
+  }
 }
 
 void csr5_export(Matrix& A, const std::string& name, std::vector<unsigned char>& out) {

@@ -25,11 +25,23 @@ constexpr int kBlock = 256;
 constexpr int kMaxClusters = 16;
 constexpr int kLdsBytes = 48 * 1024;
 
-__global__ void diag_first(int64_t m, const int* __restrict__ rp, const int* __restrict__ ci,
-                           int* __restrict__ first) {
+// first[m + c - i] = the smallest entry index on diagonal c - i (its first
+// appearance in row-major order). The stored minimum only decreases, so a
+// plain read that is already <= j (possibly an older, larger value: then the
+// atomic runs anyway) proves the atomic useless: on a banded matrix nearly all
+// of them are skipped (27-point stencil: 109M atomics on 27 addresses before).
+__global__ void diag_first(int64_t m, const int* __restrict__ rp, const int* __restrict__ ci, int* first) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  for (int j = rp[i]; j < rp[i + 1]; j++) atomicMin(first + (m - i) + ci[j], j);
+  for (int j = rp[i]; j < rp[i + 1]; j++) {
+    int* f = first + (m - i) + ci[j];
+    if (*f > j) atomicMin(f, j);
+  }
+}
+
+__global__ void fill_i32(int64_t n, int value, int* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = value;
 }
 
 __global__ void diag_flags(int64_t len, const int* __restrict__ first, int* __restrict__ flag) {
@@ -232,11 +244,10 @@ void convert_dia(Matrix& A, const lespmv_params_t& p, hipStream_t s) {
   first.alloc(sizeof(int) * std::max<int64_t>(len, 1));
   flag.alloc(sizeof(int) * std::max<int64_t>(len, 1));
   pos.alloc(sizeof(int64_t) * (len + 1));
-  {  // INT32_MAX marks an unoccupied diagonal
-    std::vector<int> init(std::max<int64_t>(len, 1), INT32_MAX);
-    LSP_HIP(hipMemcpyAsync(first.p, init.data(), sizeof(int) * init.size(), hipMemcpyHostToDevice, s));
-    LSP_HIP(hipStreamSynchronize(s));
-  }
+  // INT32_MAX marks an unoccupied diagonal
+  hipLaunchKernelGGL(fill_i32, grid1d(std::max<int64_t>(len, 1), kBlock), dim3(kBlock), 0, s,
+                     std::max<int64_t>(len, 1), INT32_MAX, first.as<int>());
+  LSP_LAUNCHED();
   if (A.m) {
     hipLaunchKernelGGL(diag_first, grid1d(A.m, kBlock), dim3(kBlock), 0, s, A.m, A.rp.as<int>(), A.ci.as<int>(),
                        first.as<int>());

@@ -3,7 +3,8 @@
 Run in the build container only (needs oracle/_ref/libref_lespmv.so, built by
 oracle/build_ref.sh from the reference sources in place). The fixture holds
 inputs and the reference's outputs: every conversion's arrays and scalars at
-the reference's own parameters (plus GPU-relevant ones), and y vectors from
+the reference's own parameters (plus GPU-relevant ones) with the reference's
+bytes_per_spmv of each converted struct, and y vectors from
 LeSpMV_<fmt> with the reference harness's alpha=0.8, beta=0.7 and y0=0
 (include/sparse_operation.h:124-138), plus y0 != 0 runs of the serial kernels.
 
@@ -109,8 +110,10 @@ def main():
             out[key(dn, name, "y0")] = y0
             for flag in (0, 1, 2):
                 y = np.zeros(A.m, dt)
-                ref.run(A.m, A.n, A.rp, A.ci, A.v, CSR, [0, 0, 0, 0], flag, 0.8, x, 0.7, y)
+                _, res = ref.run(A.m, A.n, A.rp, A.ci, A.v, CSR, [0, 0, 0, 0], flag, 0.8, x, 0.7, y)
                 out[key(dn, name, "csr", flag, "y")] = y
+            # the reference's bytes_per_spmv of the CSR struct (include/spmv_benchmark.h:43-52)
+            out[key(dn, name, "csr", "bytes_per_spmv")] = np.int64(res.scalar("bytes_per_spmv"))
             y = y0.copy()
             ref.run(A.m, A.n, A.rp, A.ci, A.v, CSR, [0, 0, 0, 0], 0, 0.8, x, 0.7, y)
             out[key(dn, name, "csr", 0, "y_beta")] = y
@@ -138,7 +141,7 @@ def main():
                         if arr == "diag_data" and a.size > 20000:
                             continue  # keep the fixture small: offsets still pin the diagonal order
                         out[key(base, arr)] = a
-                    for s in SCALARS[fmt]:
+                    for s in SCALARS[fmt] + ["bytes_per_spmv"]:
                         out[key(base, s)] = np.int64(res.scalar(s))
                     if fmt == CSR5:
                         tp = res.get("tile_ptr", np.uint32)

@@ -182,3 +182,15 @@ def test_features_file_layout(tmp_path):
     assert text[4] == "nnz_ratio 3.12346"
     header, vals = features.read_features(os.path.join(d, "mat.features"))
     assert header == "[7]: mat" and vals.shape == (40,)
+
+
+def test_image_number_format():
+    """Image files print numbers as std::ostream does by default (the reference
+    writes `file << value << std::endl`, src/sparse_features.cpp:1116-1132):
+    integers as integers, doubles like %g with six significant digits."""
+    from sspmv_amd import features
+    cases = {0.0: "0", 2.0: "2", 0.1: "0.1", 1.0 / 3.0: "0.333333", 2.0 / 3.0: "0.666667", 1234567.0: "1.23457e+06",
+             1e-5: "1e-05", 123456.0: "123456", 0.0001234565: "0.000123457", 16.5: "16.5"}
+    for v, text in cases.items():
+        assert features.fmt_value(v, False) == text, (v, features.fmt_value(v, False))
+    assert features.fmt_value(np.int32(65536), True) == "65536"

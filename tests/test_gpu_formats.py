@@ -9,7 +9,7 @@ parameters and at GPU-native ones. y must match the FP64 CSR oracle to 1e-12
 import numpy as np
 import pytest
 
-from oracle.oracle import BSR, COO, CSR5, DIA, ELL, SELL, SELL_C_R, SELL_C_SIGMA
+from oracle.oracle import BSR, COO, CSR, CSR5, DIA, ELL, SELL, SELL_C_R, SELL_C_SIGMA
 from tests.conftest import TOL, rel2
 from tests.fmt_helpers import compare_exports, lib_params, param_sets
 from tests.matrices import HostCsr, suite
@@ -166,7 +166,7 @@ def test_not_converted_is_an_error(gpu):
     assert e.value.code == gpu.ERR_NOT_CONVERTED
 
 
-@pytest.mark.parametrize("fmt", [ELL, SELL, SELL_C_SIGMA, SELL_C_R, COO, DIA, BSR])
+@pytest.mark.parametrize("fmt", [ELL, SELL, SELL_C_SIGMA, SELL_C_R, COO, DIA, BSR, CSR5])
 @pytest.mark.parametrize("k", [1, 4, 8, 16, 32])
 def test_format_spmm_columns_equal_spmv(gpu, oracle, fmt, k):
     if not available(gpu, fmt):
@@ -183,6 +183,34 @@ def test_format_spmm_columns_equal_spmv(gpu, oracle, fmt, k):
     for j in range(k):
         ref = oracle_y(oracle, A, 0.8, X[:, j].copy(), 0.7, Y0[:, j].copy())
         assert rel2(Y[:, j], ref) <= 1e-12, (fmt, k, j)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_csr5_spmm_columns_bitwise(gpu, dt):
+    """CSR5 SpMM runs the tile kernels once per right-hand side: column j of Y is
+    bit for bit the CSR5 SpMV of column j, at the reference's omega and at
+    omega = 64, with a leading dimension beyond k whose extra column stays
+    untouched; tiles that span rows and tiles of empty rows included."""
+    rng = np.random.default_rng(5)
+    for name in ("rand_long", "rand_empty", "rect_tall"):
+        A = suite(dt)[name]
+        M = gpu.Matrix.from_host(A)
+        for omega in (64 // np.dtype(dt).itemsize, 64):
+            M.convert(CSR5, gpu.make_params(omega=omega, sigma=16))
+            for k in (1, 3, 8):
+                ld = k + 1
+                X = rng.uniform(-1, 1, (A.n, ld)).astype(dt)
+                Y0 = rng.uniform(-1, 1, (A.m, ld)).astype(dt)
+                dX, dY = gpu.DeviceArray.from_numpy(X), gpu.DeviceArray.from_numpy(Y0)
+                M.spmm(CSR5, k, 0.8, dX, ld, 0.7, dY, ld)
+                Y = dY.numpy()
+                assert np.array_equal(Y[:, k:], Y0[:, k:])
+                for j in range(k):
+                    dx = gpu.DeviceArray.from_numpy(np.ascontiguousarray(X[:, j]))
+                    dy = gpu.DeviceArray.from_numpy(np.ascontiguousarray(Y0[:, j]))
+                    M.spmv(CSR5, 0.8, dx, 0.7, dy)
+                    np.testing.assert_array_equal(Y[:, j], dy.numpy(), err_msg=f"{name} omega {omega} k {k} j {j}")
+            M.release(CSR5)
 
 
 @pytest.mark.parametrize("fmt", [SELL_C_SIGMA, SELL_C_R])
@@ -203,3 +231,69 @@ def test_generated_lognormal_sliced(gpu, oracle, fmt):
         M.spmv(fmt, 1.0, dx, 0.0, dy)
         assert rel2(dy.numpy(), ref) <= 1e-12
         M.release(fmt)
+
+
+def closed_form_bytes(fmt, A, res, sv, k=1, beta_nz=False):
+    """Algorithmic bytes of one product (values + indices describing the matrix,
+    x read once, y written, y read when beta != 0), from the counts of the
+    oracle's conversion: the definitions the benchmark's roofline uses."""
+    si = 4
+    vec = k * (A.n + A.m * (2 if beta_nz else 1)) * sv
+    nnz = len(A.ci)
+    if fmt == CSR:
+        return nnz * (sv + si) + (A.m + 1) * si + vec
+    if fmt == COO:
+        return nnz * (sv + 2 * si) + vec
+    if fmt == ELL:
+        return nnz * (sv + si) + vec
+    if fmt == SELL:
+        return nnz * (sv + si) + res.scalar("chunk_num") * si + vec
+    if fmt in (SELL_C_SIGMA, SELL_C_R):
+        return nnz * (sv + si) + res.scalar("validchunkNum") * si + A.m * si + vec
+    if fmt == DIA:
+        return nnz * sv + res.scalar("complete_ndiags") * 8 + vec
+    if fmt == BSR:
+        return nnz * sv + res.scalar("nnzb") * si + (res.scalar("mb") + 1) * si + vec
+    if fmt == CSR5:
+        p = res.scalar("p")
+        return (nnz * (sv + si) + (p + 1) * 4 + p * res.scalar("omega") * res.scalar("num_packets") * 4
+                + res.scalar("num_offsets") * si + vec)
+    raise ValueError(fmt)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_byte_formulas(gpu, oracle, golden, dt):
+    """lespmv_reference_bytes equals the compiled reference's bytes_per_spmv
+    (include/spmv_benchmark.h:26-150, stored in the fixture for every matrix,
+    format and parameter set); lespmv_effective_bytes equals the closed forms
+    above for k = 1 and 4, beta zero and not."""
+    dn = np.dtype(dt).name
+    sv = np.dtype(dt).itemsize
+    names = sorted({k.split("/")[1] for k in golden.files if k.startswith(dn + "/") and k.endswith("/m")})
+    checked = 0
+    for name in names:
+        g = lambda s: golden[f"{dn}/{name}/{s}"]
+        A = HostCsr(int(g("m")), int(g("n")), g("rp"), g("ci"), g("v"))
+        M = gpu.Matrix.from_host(A)
+        assert M.reference_bytes(CSR) == int(g("csr/bytes_per_spmv")), name
+        for k, bz in ((1, False), (1, True), (4, True)):
+            assert M.effective_bytes(CSR, k, bz) == closed_form_bytes(CSR, A, None, sv, k, bz), (name, k, bz)
+        for fmt in FORMATS:
+            pi = 0
+            while f"{dn}/{name}/{fmt}/{pi}/params" in golden.files:
+                base = f"{dn}/{name}/{fmt}/{pi}"
+                pi += 1
+                if int(golden[f"{base}/rc"]) != 0:
+                    continue
+                prm = golden[f"{base}/params"]
+                rc, res = oracle.convert(A.m, A.n, A.rp, A.ci, A.v, fmt, prm)
+                assert rc == 0, base
+                M.convert(fmt, lib_params(gpu, fmt, prm))
+                assert M.reference_bytes(fmt) == int(golden[f"{base}/bytes_per_spmv"]), base
+                for k, bz in ((1, False), (1, True), (4, False)):
+                    want = closed_form_bytes(fmt, A, res, sv, k, bz)
+                    assert M.effective_bytes(fmt, k, bz) == want, (base, k, bz)
+                M.release(fmt)
+                checked += 1
+    assert checked > 50
+    print(f"{dn}: byte formulas checked on {checked} conversions")
