@@ -275,8 +275,12 @@ typedef struct {
 /* The standard grid for the formats whose bit is set in fmt_mask (bit = format
  * id): CSR lanes per row 2..64 and merge-path, COO, ELL, S-ELL / SELL-C-sigma
  * (sigma 256, 4096) / SELL-C-R with C in {32, 64}, DIA both variants, BSR
- * 2x2 4x4 8x4 8x8 16x4, CSR5 (omega, sigma) in {(64,16), (32,16), (64,8)}.
- * With out == NULL only *count is set. */
+ * 2x2 4x4 8x4 8x8 16x4, CSR5 (omega, sigma) in {(64,16), (32,16), (64,8)};
+ * with LESPMV_TUNE_PANELS also the CSR column panels (8 and 4 panels), after
+ * all the others (their layout is built with lespmv_prepare, timed as the
+ * candidate's convert_ms, and released again). With out == NULL only *count
+ * is set. */
+#define LESPMV_TUNE_PANELS (1 << LESPMV_NUM_FORMATS)
 int lespmv_tune_candidates(int32_t fmt_mask, int32_t dtype, lespmv_tune_result_t* out, int32_t cap, int32_t* count);
 int lespmv_tune(lespmv_matrix_t A, lespmv_tune_result_t* candidates, int32_t n, int32_t iters, int32_t k);
 /* Best candidate of one format's grid; the winner stays converted on A. */
@@ -426,18 +430,19 @@ int lespmv_block_counts(lespmv_matrix_t A, const int32_t* dims, int32_t n, int64
  * SpMV_Adapter.py:90-122) with an MLP trained on this library's own timings.
  * 128 inputs from the GPU features (the 40 .features values, 24 derived
  * scalars incl. BSR block fills, 32-bin row-block and column-block shares).
- * Six heads (32 outputs): format class {CSR-vector, CSR-merge, COO, ELL,
- * SELL family, DIA, BSR, CSR5}, CSR lanes, SELL (C, sigma), DIA variant, BSR
- * block, CSR5 (omega, sigma); a class's parameter head indexes its run
- * [class_first[c], class_first[c+1]) of the standard tuning grid (all formats).
+ * Seven heads (35 outputs): format class {CSR-vector, CSR-merge, COO, ELL,
+ * SELL family, DIA, BSR, CSR5, CSR column panels}, CSR lanes, SELL (C, sigma),
+ * DIA variant, BSR block, CSR5 (omega, sigma), panels (8, 4); a class's
+ * parameter head indexes its run [class_first[c], class_first[c+1]) of the
+ * standard tuning grid (all formats and LESPMV_TUNE_PANELS).
  * allowed[32] marks what the features already rule out (ELL / DIA fill,
  * diagonal count, BSR fill of the counted block sizes). */
 #define LESPMV_SELECTOR_INPUTS 128
-#define LESPMV_SELECTOR_OUTPUTS 32
-int lespmv_selector_layout(int32_t* class_first /*[9]*/, int32_t* head_of_class /*[8]*/, int32_t* head_sizes /*[6]*/);
-int lespmv_selector_inputs(lespmv_matrix_t A, int32_t flags, float* inputs /*[128]*/, uint8_t* allowed /*[32]*/,
+#define LESPMV_SELECTOR_OUTPUTS 35
+int lespmv_selector_layout(int32_t* class_first /*[10]*/, int32_t* head_of_class /*[9]*/, int32_t* head_sizes /*[7]*/);
+int lespmv_selector_inputs(lespmv_matrix_t A, int32_t flags, float* inputs /*[128]*/, uint8_t* allowed /*[35]*/,
                            lespmv_features_t* features /* may be NULL */);
-int lespmv_selector_decode(const float* probs /*[32]*/, const uint8_t* allowed /*[32]*/, int32_t* cls,
+int lespmv_selector_decode(const float* probs /*[35]*/, const uint8_t* allowed /*[35]*/, int32_t* cls,
                            int32_t* candidate);
 /* Features, forward pass and decode: the chosen grid candidate (not converted). */
 int lespmv_select(lespmv_matrix_t A, lespmv_mlp_t mlp, int32_t flags, lespmv_tune_result_t* choice, int32_t* cls);

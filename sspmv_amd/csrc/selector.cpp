@@ -6,10 +6,11 @@
 // format labels (LeSpMV/script/write_label.py:63-75).
 //
 // Heads: 0 = format class (CSR-vector, CSR-merge, COO, ELL, SELL family, DIA,
-// BSR, CSR5), 1 = CSR lanes per row (2..64), 2 = SELL (C, sigma): C in {32, 64}
-// x {S-ELL, sigma 256, sigma 4096, SELL-C-R}, 3 = DIA variant, 4 = BSR block
-// (2x2, 4x4, 8x4, 8x8, 16x4), 5 = CSR5 (omega, sigma). A class's parameter
-// head indexes its run of candidates in the grid.
+// BSR, CSR5, CSR column panels), 1 = CSR lanes per row (2..64), 2 = SELL
+// (C, sigma): C in {32, 64} x {S-ELL, sigma 256, sigma 4096, SELL-C-R},
+// 3 = DIA variant, 4 = BSR block (2x2, 4x4, 8x4, 8x8, 16x4), 5 = CSR5
+// (omega, sigma), 6 = panels (8, 4). A class's parameter head indexes its run
+// of candidates in the grid (all formats plus LESPMV_TUNE_PANELS).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -25,12 +26,14 @@ using namespace lsp;
 
 namespace {
 
-constexpr int kClasses = 8, kHeads = 6, kOut = 32;
-// first grid index of each class (the grid order of lespmv_tune_candidates with all formats)
-constexpr int kClassFirst[kClasses + 1] = {0, 6, 7, 8, 9, 17, 19, 24, 27};
-constexpr int kHeadOfClass[kClasses] = {1, -1, -1, -1, 2, 3, 4, 5};
-constexpr int kHeadSizes[kHeads] = {8, 6, 8, 2, 5, 3};
-constexpr int kHeadOff[kHeads] = {0, 8, 14, 22, 24, 29};
+constexpr int kClasses = 9, kHeads = 7, kOut = LESPMV_SELECTOR_OUTPUTS;
+// first grid index of each class (the grid order of lespmv_tune_candidates with
+// all formats and LESPMV_TUNE_PANELS)
+constexpr int kClassFirst[kClasses + 1] = {0, 6, 7, 8, 9, 17, 19, 24, 27, 29};
+constexpr int kHeadOfClass[kClasses] = {1, -1, -1, -1, 2, 3, 4, 5, 6};
+constexpr int kHeadSizes[kHeads] = {9, 6, 8, 2, 5, 3, 2};
+constexpr int kHeadOff[kHeads] = {0, 9, 15, 23, 25, 30, 33};
+static_assert(kHeadOff[kHeads - 1] + kHeadSizes[kHeads - 1] == kOut, "head layout");
 constexpr int kBlockDims[3][2] = {{2, 2}, {4, 4}, {8, 8}};
 constexpr int kBsrKnown[3] = {0, 1, 3};  // positions of 2x2, 4x4, 8x8 in the BSR head
 constexpr double kMaxFill = 4.0;          // the grid's params.max_fill
@@ -186,9 +189,10 @@ extern "C" int lespmv_select(lespmv_matrix_t h, lespmv_mlp_t mlp, int32_t flags,
     int32_t dtype = 0;
     lespmv_matrix_info(h, &m, &n, &nnz, &dtype);
     int32_t count = 0;
-    lespmv_tune_candidates((1 << LESPMV_NUM_FORMATS) - 1, dtype, nullptr, 0, &count);
+    const int32_t mask = ((1 << LESPMV_NUM_FORMATS) - 1) | LESPMV_TUNE_PANELS;
+    lespmv_tune_candidates(mask, dtype, nullptr, 0, &count);
     std::vector<lespmv_tune_result_t> grid(count);
-    lespmv_tune_candidates((1 << LESPMV_NUM_FORMATS) - 1, dtype, grid.data(), count, &count);
+    lespmv_tune_candidates(mask, dtype, grid.data(), count, &count);
     LSP_REQUIRE(cand >= 0 && cand < count, LESPMV_ERR_DEVICE, "selector layout and tuning grid disagree");
     *choice = grid[cand];
     if (cls) *cls = c;

@@ -94,7 +94,12 @@ __global__ __launch_bounds__(kBlock) void bsr_row_kernel(int64_t m, int64_t n, i
   y[i] = beta == V(0) ? alpha * sum : alpha * sum + beta * y[i];
 }
 
-// One wavefront per block row; BS = br*bc divides 64, BC = bc.
+// One wavefront per block row; BS = br*bc divides 64, BC = bc. G = 64 / BS
+// blocks per step; U steps' block columns and values are loaded (streamed
+// past the caches that keep x) before any of their x gathers, so a block row
+// of the 27-point stencil (about 26 blocks of 4 x 4) has all its loads in
+// flight at once. Workgroups are renumbered so each XCD takes a contiguous
+// range of block rows (their x neighbourhoods stay in that XCD's L2).
 template <class V, int BS, int BC>
 __global__ __launch_bounds__(kBlock) void bsr_wave_kernel(int64_t m, int64_t n, int64_t mb,
                                                           const int* __restrict__ rptr, const int* __restrict__ bcol,
@@ -102,15 +107,28 @@ __global__ __launch_bounds__(kBlock) void bsr_wave_kernel(int64_t m, int64_t n, 
                                                           V* __restrict__ y, V alpha, V beta) {
   constexpr int G = 64 / BS;  // blocks per wave step
   constexpr int BR = BS / BC;
+  constexpr int U = BS <= 16 ? 8 : 4;  // steps in flight
   const int lane = threadIdx.x & 63;
-  const int64_t b = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
   if (b >= mb) return;
   const int q = lane / BS, p = lane % BS, c = p % BC;
   const int k0 = rptr[b], k1 = rptr[b + 1];
   V acc = 0;
-  for (int k = k0 + q; k < k1; k += G) {
-    const int64_t xc = (int64_t)bcol[k] * BC + c;
-    if (xc < n) acc += data[(int64_t)k * BS + p] * x[xc];
+  for (int kb = k0 + q; kb < k1; kb += G * U) {
+    int col[U];
+    V a[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = kb + u * G;
+      const bool ok = k < k1;
+      col[u] = ok ? __builtin_nontemporal_load(bcol + k) : -1;
+      a[u] = ok ? __builtin_nontemporal_load(data + (int64_t)k * BS + p) : V(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t xc = (int64_t)col[u] * BC + c;
+      if (col[u] >= 0 && xc < n) acc += a[u] * x[xc];
+    }
   }
   // sum over the BC columns of a block row, then over the G blocks
 #pragma unroll

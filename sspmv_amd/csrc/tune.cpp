@@ -77,7 +77,18 @@ std::vector<lespmv_tune_result_t> grid(int32_t fmt_mask, int dtype) {
       p.sigma = ws.second;
       c.push_back(candidate(LESPMV_CSR5, LESPMV_VARIANT_AUTO, p));
     }
+  // the CSR column panels (8 and 4 panels), last so that the other candidates
+  // keep their places in the grid
+  if ((fmt_mask & LESPMV_TUNE_PANELS) != 0) {
+    c.push_back(candidate(LESPMV_CSR, LESPMV_VARIANT_PANEL, base));
+    c.push_back(candidate(LESPMV_CSR, LESPMV_VARIANT_PANEL4, base));
+  }
   return c;
+}
+
+bool is_panel(const lespmv_tune_result_t& r) {
+  return r.fmt == LESPMV_CSR && (r.variant == LESPMV_VARIANT_PANEL || r.variant == LESPMV_VARIANT_PANEL16 ||
+                                 r.variant == LESPMV_VARIANT_PANEL4);
 }
 
 // Converts (unless CSR), times `iters` calls after two warm-up calls, and
@@ -90,6 +101,10 @@ void time_one(lespmv_matrix_t h, lespmv_tune_result_t& r, int32_t iters, int32_t
     r.status = lespmv_convert(h, r.fmt, &r.params);
     if (r.status != LESPMV_OK) return;
     lespmv_convert_ms(h, r.fmt, &r.convert_ms);
+  } else if (is_panel(r)) {  // the panel layout, built and timed like a conversion
+    r.status = lespmv_prepare(h, LESPMV_CSR, r.variant);
+    if (r.status != LESPMV_OK) return;
+    lespmv_convert_ms(h, LESPMV_CSR, &r.convert_ms);
   }
   double ms = 0.0;
   int rc;
@@ -104,7 +119,7 @@ void time_one(lespmv_matrix_t h, lespmv_tune_result_t& r, int32_t iters, int32_t
   (void)m;
   r.status = rc;
   if (rc == LESPMV_OK) r.ms = ms;
-  if (!keep && r.fmt != LESPMV_CSR) lespmv_release_format(h, r.fmt);
+  if (!keep && (r.fmt != LESPMV_CSR || is_panel(r))) lespmv_release_format(h, r.fmt);
 }
 
 }  // namespace

@@ -447,6 +447,7 @@ class Matrix:
 
 
 ALL_FORMATS_MASK = (1 << 9) - 1
+TUNE_PANELS = 1 << 9  # adds the CSR column-panel candidates after all others (LESPMV_TUNE_PANELS)
 
 
 def tune_candidates(fmt_mask: int = ALL_FORMATS_MASK, dtype=np.float64):
@@ -533,18 +534,21 @@ class Mlp:
 
 
 def selector_layout():
-    cf = np.zeros(9, dtype=np.int32)
-    hc = np.zeros(8, dtype=np.int32)
-    hs = np.zeros(6, dtype=np.int32)
+    cf = np.zeros(10, dtype=np.int32)
+    hc = np.zeros(9, dtype=np.int32)
+    hs = np.zeros(7, dtype=np.int32)
     check(load().lespmv_selector_layout(cf.ctypes.data_as(C.c_void_p), hc.ctypes.data_as(C.c_void_p),
                                         hs.ctypes.data_as(C.c_void_p)))
     return cf, hc, hs
 
 
+SELECTOR_OUTPUTS = 35  # LESPMV_SELECTOR_OUTPUTS
+
+
 def selector_inputs(M: "Matrix", symmetric: bool = False):
-    """(128 float inputs, 32 allowed flags, features dict) of a handle."""
+    """(128 float inputs, 35 allowed flags, features dict) of a handle."""
     x = np.zeros(128, dtype=np.float32)
-    allowed = np.zeros(32, dtype=np.uint8)
+    allowed = np.zeros(SELECTOR_OUTPUTS, dtype=np.uint8)
     f = FeaturesC()
     check(load().lespmv_selector_inputs(M.h, FEAT_SYMMETRIC_FILE if symmetric else 0, x.ctypes.data_as(C.c_void_p),
                                         allowed.ctypes.data_as(C.c_void_p), C.byref(f)))

@@ -30,8 +30,12 @@ import numpy as np
 from . import lib
 
 FAMILIES = ["banded", "powerlaw", "blockdiag", "uniform", "lognormal", "blockrand", "laplace2d", "stencil27"]
-CLASS_NAMES = ["CSR-vector", "CSR-merge", "COO", "ELL", "SELL", "DIA", "BSR", "CSR5"]
-N_GRID = 27  # candidates of the standard grid; index 27 is the default CSR kernel
+CLASS_NAMES = ["CSR-vector", "CSR-merge", "COO", "ELL", "SELL", "DIA", "BSR", "CSR5", "CSR-panel"]
+N_CLASSES = len(CLASS_NAMES)
+# candidates of the standard grid (all formats plus the two CSR column-panel
+# variants, grid indices 27 and 28); times[:, N_GRID] is the default CSR kernel
+N_GRID = 29
+GRID_MASK = lib.ALL_FORMATS_MASK | lib.TUNE_PANELS
 
 
 def sample_matrix(rng: np.random.Generator, max_nnz: float, n_range=(1e4, 1e6)):
@@ -79,7 +83,7 @@ def collect(count: int, out: str, seed: int = 1, iters: int = 10, max_nnz: float
             log=print):
     """Time the grid on `count` generated matrices; writes an .npz timing database."""
     rng = np.random.default_rng(seed)
-    grid = lib.tune_candidates()
+    grid = lib.tune_candidates(GRID_MASK)
     rows = {k: [] for k in ("X", "allowed", "times", "convert_ms", "family", "m", "nnz", "seed")}
     t0 = time.time()
     for i in range(count):
@@ -122,11 +126,11 @@ def load_db(path):
     # inputs written before non-finite values were mapped to 0 in
     # lespmv_selector_inputs (empty matrices): the same mapping
     db["X"] = np.where(np.isfinite(db["X"]), db["X"], 0.0).astype(np.float32)
-    return db
+    return widen_db(db)
 
 
 def labels(times: np.ndarray, tie: float = 0.0) -> np.ndarray:
-    """(n, 6) labels: class of the fastest grid candidate, then per parameter
+    """(n, 7) labels: class of the fastest grid candidate, then per parameter
     head the fastest candidate of that class (-1 when none of it applies).
     With tie > 0, candidates within a factor (1 + tie) of the fastest count as
     equally fast and the label goes to the one whose class wins most often
@@ -134,17 +138,17 @@ def labels(times: np.ndarray, tie: float = 0.0) -> np.ndarray:
     cf, hc, hs = lib.selector_layout()
     g = times[:, :N_GRID]
     cls_of = np.zeros(N_GRID, dtype=np.int32)
-    for c in range(8):
+    for c in range(N_CLASSES):
         cls_of[cf[c]:cf[c + 1]] = c
-    L = np.full((len(g), 6), -1, dtype=np.int32)
+    L = np.full((len(g), len(hs)), -1, dtype=np.int32)
     best = np.argmin(g, axis=1)
     if tie > 0:
-        prior = np.bincount(cls_of[best], minlength=8).astype(np.float64)
+        prior = np.bincount(cls_of[best], minlength=N_CLASSES).astype(np.float64)
         near = g <= (1.0 + tie) * g[np.arange(len(g)), best][:, None]
         score = np.where(near, prior[cls_of][None, :] - g / g.max(), -np.inf)
         best = np.argmax(score, axis=1)
     L[:, 0] = cls_of[best]
-    for c in range(8):
+    for c in range(N_CLASSES):
         h = hc[c]
         if h < 0:
             continue
@@ -160,8 +164,8 @@ REFERENCE_LABELS = ["COO", "CSR", "DIA", "ELL", "S-ELL", "S-ELL-sigma", "S-ELL-R
 
 def reference_format_times(times: np.ndarray) -> np.ndarray:
     """(n, 9) best time per reference format label (inf where none applies):
-    every grid candidate belongs to one label (both CSR kernels to CSR)."""
-    grid = lib.tune_candidates()
+    every grid candidate belongs to one label (every CSR kernel to CSR)."""
+    grid = lib.tune_candidates(GRID_MASK)
     fmt_label = {lib.COO: 0, lib.CSR: 1, lib.DIA: 2, lib.ELL: 3, lib.SELL: 4, lib.SELL_C_SIGMA: 5,
                  lib.SELL_C_R: 6, lib.CSR5: 7, lib.BSR: 8}
     out = np.full((len(times), 9), np.inf)
@@ -279,7 +283,7 @@ def run_training(db_path, model_out, report_out=None, epochs=400, seed=7, tie=0.
         "rule_dia_else_csr": evaluate(times[te], rule_pick({k: v[te] for k, v in db.items()})),
         "best_fixed_candidate": best_fixed(times[tr], times[te]),
         "train_selector": evaluate(times[tr], picks(mlp, X[tr], db["allowed"][tr])),
-        "class_histogram": np.bincount(L[:, 0], minlength=8).tolist(),
+        "class_histogram": np.bincount(L[:, 0], minlength=N_CLASSES).tolist(),
         "class_names": CLASS_NAMES,
     }
     if report_out:
@@ -307,6 +311,68 @@ def run_evaluation(db_path, model_path, report_out=None, log=print):
             json.dump(res, f, indent=1)
     log(json.dumps(res), flush=True)
     return res
+
+
+def widen_db(db):
+    """A database timed on the 27-candidate grid (times: 27 + the default CSR
+    kernel; allowed: 32 flags) laid out for the grid with the two panel
+    candidates: their columns (inf until timed) go before the default, and the
+    allowed flags gain the panel class and its two parameters."""
+    t = db["times"]
+    if t.shape[1] == N_GRID + 1:
+        return db
+    assert t.shape[1] == 28, t.shape
+    n = len(t)
+    out = dict(db)
+    inf2 = np.full((n, 2), np.inf)
+    out["times"] = np.concatenate([t[:, :27], inf2, t[:, 27:]], axis=1)
+    out["convert_ms"] = np.concatenate([db["convert_ms"][:, :27], np.zeros((n, 2)), db["convert_ms"][:, 27:]],
+                                       axis=1)
+    a = db["allowed"]
+    ones = np.ones((n, 1), np.uint8)
+    out["allowed"] = np.concatenate([a[:, :8], ones, a[:, 8:], ones, ones], axis=1)
+    return out
+
+
+def retime_panels(db_path, out, iters=10, max_nnz=6e6, n_max=2e6, rows=None, log=print):
+    """Times the two CSR column-panel candidates on the matrices of a timing
+    database collected before they joined the grid (all rows, or the rows
+    [start, stop) given, which alone are written to `out`). The matrices are
+    made again from their stored seeds (collection seed * 1,000,003 + index:
+    the sampler is replayed from that collection's start, and the regenerated
+    sizes and family must equal the stored ones), so the other columns stay
+    as they were timed."""
+    db = widen_db(load_db(db_path))
+    if rows is not None:
+        db = {k: v[rows[0]:rows[1]] for k, v in db.items()}
+    grid = lib.tune_candidates(GRID_MASK)
+    seeds, idx = np.divmod(db["seed"], 1_000_003)
+    rngs, pos = {}, {}
+    t0 = time.time()
+    for r in range(len(seeds)):
+        s, i = int(seeds[r]), int(idx[r])
+        if s not in rngs:
+            rngs[s], pos[s] = np.random.default_rng(s), 0
+        assert pos[s] <= i, f"rows of seed {s} out of order"
+        while pos[s] < i:  # matrices of this collection before the first row taken
+            sample_matrix(rngs[s], max_nnz, (1e4, n_max))
+            pos[s] += 1
+        fam, gen, params = sample_matrix(rngs[s], max_nnz, (1e4, n_max))
+        pos[s] += 1
+        A = lib.generate(gen, params, int(db["seed"][r]))
+        assert (A.m, A.nnz, FAMILIES.index(fam)) == (db["m"][r], db["nnz"][r], db["family"][r]), r
+        M = lib.Matrix.from_host(A)
+        cands = (lib.TuneResultC * 2)()
+        cands[0], cands[1] = grid[27], grid[28]
+        M.tune(cands, iters=iters)
+        for j in range(2):
+            db["times"][r, 27 + j] = cands[j].ms if cands[j].status == 0 else np.inf
+            db["convert_ms"][r, 27 + j] = cands[j].convert_ms
+        del M, A
+        if (r + 1) % 500 == 0 or r + 1 == len(seeds):
+            log(f"retime {r + 1}/{len(seeds)} {time.time() - t0:.0f}s", flush=True)
+    np.savez_compressed(out, **db)
+    return db
 
 
 class Selector:
@@ -338,6 +404,12 @@ def main(argv=None):
     t.add_argument("--tie", type=float, default=0.0)
     t.add_argument("--weight-decay", type=float, default=1e-4)
     t.add_argument("--hidden", type=int, default=128)
+    rp = sub.add_parser("retime-panels", help="time the CSR column-panel candidates into an older database")
+    rp.add_argument("--db", required=True)
+    rp.add_argument("--out", required=True)
+    rp.add_argument("--iters", type=int, default=10)
+    rp.add_argument("--n-max", type=float, default=2e6)
+    rp.add_argument("--rows", help="START:STOP, the rows to time (written alone to --out)")
     e = sub.add_parser("evaluate")
     e.add_argument("--db", required=True)
     e.add_argument("--model", required=True)
@@ -346,6 +418,9 @@ def main(argv=None):
     lib.check(lib.load().lespmv_init(0))
     if a.cmd == "collect":
         collect(a.count, a.out, a.seed, a.iters, a.max_nnz, (1e4, a.n_max))
+    elif a.cmd == "retime-panels":
+        rows = tuple(int(v) for v in a.rows.split(":")) if a.rows else None
+        retime_panels(a.db, a.out, a.iters, n_max=a.n_max, rows=rows)
     elif a.cmd == "evaluate":
         run_evaluation(a.db, a.model, a.report)
     else:

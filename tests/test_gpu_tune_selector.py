@@ -68,8 +68,10 @@ def test_selector_small_pipeline(gpu, tmp_path):
     lib = gpu
     path = str(tmp_path / "db.npz")
     db = selector.collect(24, path, seed=5, iters=2, max_nnz=2e5, n_range=(2e3, 2e4), log=lambda *a, **k: None)
-    assert db["X"].shape == (24, 128) and db["times"].shape == (24, 28)
-    assert np.isfinite(db["times"][:, :27].min(axis=1)).all()
+    assert db["X"].shape == (24, 128) and db["times"].shape == (24, selector.N_GRID + 1)
+    assert db["allowed"].shape == (24, lib.SELECTOR_OUTPUTS)
+    assert np.isfinite(db["times"][:, :selector.N_GRID].min(axis=1)).all()
+    assert np.isfinite(db["times"][:, 27:29]).all()  # both panel candidates timed
     L = selector.labels(db["times"])
     assert (L[:, 0] >= 0).all()
     mlp, hist = selector.train(db["X"], L, epochs=30, seed=1)
@@ -85,7 +87,7 @@ def test_selector_small_pipeline(gpu, tmp_path):
     np.testing.assert_array_equal(x, db["X"][0])
     choice, cls = lib.select(M, mlp)
     c2, cand2 = lib.selector_decode(mlp.forward(x[None, :])[0], allowed)
-    grid = lib.tune_candidates()
+    grid = lib.tune_candidates(selector.GRID_MASK)
     assert cls == c2 and choice.fmt == grid[cand2].fmt and choice.variant == grid[cand2].variant
 
 
