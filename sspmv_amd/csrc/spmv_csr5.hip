@@ -77,55 +77,52 @@ __global__ void desc_bits(int64_t m, int64_t p, int64_t tile, int omega, int sig
   atomicOr(desc + t * omega * packets + (int64_t)ly * omega + lx, 1u << (31 - llid));
 }
 
-// step 2.2: y_offset and scansum_offset in the first packet (:1140-1245).
-// One thread per tile; also the per-tile count of descriptor offsets.
-__global__ void desc_offsets(int64_t p, int omega, int sigma, int packets, int bit_y, int bit_all,
-                             const uint32_t* __restrict__ tp, uint32_t* __restrict__ desc,
-                             int64_t* __restrict__ off_count) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p) return;
-  off_count[t] = 0;
+// step 2.2: y_offset and scansum_offset in the first packet (:1140-1245), and
+// the per-tile count of descriptor offsets. One group of W = omega lanes per
+// tile, lane l on its own descriptor word (one packet: bit_all + sigma <= 32).
+// Lane l's segments: the row starts at its positions 1..sigma-1 (a popcount),
+// minus one when position 0 starts no row, plus one when any position does
+// (lane 0 always counts as starting); y_offset is the exclusive scan of the
+// segments (less one, 0 for lane 0, in unsigned arithmetic), scansum_offset
+// the number of lanes without a row start up to the next lane with one.
+template <int W>
+__global__ __launch_bounds__(kBlock) void desc_offsets(int64_t p, int sigma, int bit_y, int bit_all,
+                                                       const uint32_t* __restrict__ tp, uint32_t* __restrict__ desc,
+                                                       int64_t* __restrict__ off_count) {
+  const int l = threadIdx.x & (W - 1);
+  const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / W;
+  if (t >= p) return;  // whole W-lane groups leave together
+  if (l == 0) off_count[t] = 0;
   if (t >= p - 1) return;
-  const bool empty_rows = (tp[t] >> 31) & 1;
-  if ((tp[t] & kMask) == (tp[t + 1] & kMask)) return;
-  uint32_t* dt = desc + t * omega * packets;
-  int segn[65], present[65];
-  for (int lane = 0; lane < omega; lane++) {
-    int start, stop = 0, pres = !lane, ly = 0;
-    uint32_t bitflag = (dt[lane] << bit_all) | ((uint32_t)pres << 31);
-    start = !((bitflag >> 31) & 1);
-    pres |= (bitflag >> 31) & 1;
-    for (int i = 1; i < sigma; i++) {
-      if ((!ly && i == 32 - bit_all) || (ly && (i - (32 - bit_all)) % 32 == 0)) {
-        ly++;
-        bitflag = dt[ly * omega + lane];
-      }
-      const int norm_i = !ly ? i : i - (32 - bit_all);
-      stop += (bitflag >> (31 - norm_i % 32)) & 1;
-      pres |= (bitflag >> (31 - norm_i % 32)) & 1;
-    }
-    const int s = stop - start + pres;
-    segn[lane] = s > 0 ? s : 0;
-    present[lane] = pres;
+  const uint32_t a = tp[t], b = tp[t + 1];
+  if ((a & kMask) == (b & kMask)) return;  // the tile lies inside one row
+  uint32_t* dt = desc + t * W;
+  const uint32_t d = dt[l];
+  const uint32_t flag = (d << bit_all) | ((uint32_t)(l == 0) << 31);  // bit 31 - i <-> position i
+  const uint32_t rest = (flag >> (32 - sigma)) & ((1u << (sigma - 1)) - 1u);  // positions 1..sigma-1
+  const int first = (flag >> 31) & 1;
+  const int pres = first | (rest != 0);
+  const int cnt = __popc(rest) - !first + pres;
+  const int seg = cnt > 0 ? cnt : 0;
+  int inc = seg;
+#pragma unroll
+  for (int o = 1; o < W; o <<= 1) {
+    const int v = __shfl_up(inc, o, W);
+    if (l >= o) inc += v;
   }
-  segn[omega] = 0;
-  present[omega] = 1;
-  int acc = 0;
-  for (int i = 0; i <= omega; i++) {
-    const int v = segn[i];
-    segn[i] = acc;
-    acc += v;
+  const int total = __shfl(inc, W - 1, W);
+  const uint64_t bal = __ballot(pres);
+  const int gbase = (threadIdx.x & 63) & ~(W - 1);
+  const uint64_t gm = (bal >> gbase) & (W == 64 ? ~0ull : ((1ull << W) - 1));
+  uint32_t scan_off = 0;
+  if (pres) {
+    const uint64_t after = l + 1 < W ? gm >> (l + 1) : 0;
+    const int next = after ? l + __ffsll((unsigned long long)after) : W;  // next lane with a row start
+    scan_off = (uint32_t)(next - l - 1);
   }
-  if (empty_rows) off_count[t] = segn[omega];
-  for (int lane = 0; lane < omega; lane++) {
-    uint32_t y_off = (uint32_t)segn[lane];
-    uint32_t scan_off = 0;
-    int next1 = lane + 1;
-    if (present[lane])
-      while (next1 < omega && !present[next1]) { scan_off++; next1++; }
-    y_off = lane ? y_off - 1 : 0;
-    dt[lane] |= (y_off << (32 - bit_y)) | (scan_off << (32 - bit_all));
-  }
+  const uint32_t y_off = l ? (uint32_t)(inc - seg) - 1u : 0u;
+  dt[l] = d | (y_off << (32 - bit_y)) | (scan_off << (32 - bit_all));
+  if (l == 0 && ((a >> 31) & 1)) off_count[t] = total;
 }
 
 // tile_desc_offset (:1250-1361): one thread per (tile, lane) of tiles with empty rows
@@ -207,6 +204,10 @@ __global__ void scale_list(int64_t n, const int* __restrict__ list, V* __restric
 
 // ---- SpMV -------------------------------------------------------------------
 
+// The strided instantiations (S, SpMM one column at a time) keep some of the
+// 32-slot loops rolled (the compiler reports it); the SpMV ones unroll fully.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wpass-failed"
 template <class V, int W, bool S>
 __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int bit_y, int bit_all,
                                                      const uint32_t* __restrict__ tp,
@@ -216,8 +217,6 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
                                                      const V* __restrict__ x, V* __restrict__ y,
                                                      V* __restrict__ calib, V alpha, V beta, int64_t ldx,
                                                      int64_t ldy) {
-  // y row r of this column
-  auto Y = [&](int64_t r) -> V& { return y[at<S>(r, ldy)]; };
   const int l = threadIdx.x & (W - 1);
   const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / W;
   if (t >= p - 1) return;  // whole W-lane groups leave together
@@ -250,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
     for (int o = W / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, W);
     if (l == 0) {
       if (first_direct) {
-        Y(rs) = beta == V(0) ? alpha * s : alpha * s + beta * Y(rs);
+        y[at<S>(rs, ldy)] = beta == V(0) ? alpha * s : alpha * s + beta * y[at<S>(rs, ldy)];
         calib[t] = 0;
       } else {
         calib[t] = s;
@@ -277,12 +276,12 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
         head = acc;
         seg = (l == 0) ? 0 : (int)y_off + 1;  // lane 0 opens segment 0 at position 0
       } else if (seg == 0) {
-        if (first_direct) Y(rs) = beta == V(0) ? alpha * acc : alpha * acc + beta * Y(rs);
+        if (first_direct) y[at<S>(rs, ldy)] = beta == V(0) ? alpha * acc : alpha * acc + beta * y[at<S>(rs, ldy)];
         else calib[t] = acc;
         seg = 1;
       } else {
         const int64_t r = row_of(seg);
-        Y(r) = beta == V(0) ? alpha * acc : alpha * acc + beta * Y(r);
+        y[at<S>(r, ldy)] = beta == V(0) ? alpha * acc : alpha * acc + beta * y[at<S>(r, ldy)];
         seg++;
       }
       acc = 0;
@@ -311,14 +310,16 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
   if (!has_flag) return;
   const V total = acc + (l + 1 < W ? carry_all : V(0));
   if (seg == 0) {
-    if (first_direct) Y(rs) = beta == V(0) ? alpha * total : alpha * total + beta * Y(rs);
+    if (first_direct) y[at<S>(rs, ldy)] = beta == V(0) ? alpha * total : alpha * total + beta * y[at<S>(rs, ldy)];
     else calib[t] = total;
   } else {
     const int64_t r = row_of(seg);
-    Y(r) = beta == V(0) ? alpha * total : alpha * total + beta * Y(r);
+    y[at<S>(r, ldy)] = beta == V(0) ? alpha * total : alpha * total + beta * y[at<S>(r, ldy)];
   }
   if (l == 0 && first_direct) calib[t] = 0;
 }
+
+#pragma clang diagnostic pop
 
 // Continuation rows: the first continuation tile of a chain adds the
 // calibrators of the chain in tile order (reference calibrate kernel, :367-384).
@@ -440,9 +441,20 @@ void convert_csr5(Matrix& A, const lespmv_params_t& prm, hipStream_t s) {
   LSP_LAUNCHED();
   DBuf cnt;
   cnt.alloc(sizeof(int64_t) * (p + 1));
-  hipLaunchKernelGGL(desc_offsets, grid1d(p, 64), dim3(64), 0, s, p, omega, sigma, d->packets, d->bit_y, bit_all,
-                     d->tile_ptr.as<uint32_t>(), d->tile_desc.as<uint32_t>(), cnt.as<int64_t>());
-  LSP_LAUNCHED();
+  {
+    auto offsets = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid1d(p * omega, kBlock), dim3(kBlock), 0, s, p, sigma, d->bit_y, bit_all,
+                         d->tile_ptr.as<uint32_t>(), d->tile_desc.as<uint32_t>(), cnt.as<int64_t>());
+    };
+    switch (omega) {  // omega is a power of two in [4, 64], checked above
+      case 4: offsets(desc_offsets<4>); break;
+      case 8: offsets(desc_offsets<8>); break;
+      case 16: offsets(desc_offsets<16>); break;
+      case 32: offsets(desc_offsets<32>); break;
+      default: offsets(desc_offsets<64>); break;
+    }
+    LSP_LAUNCHED();
+  }
   d->num_offsets = scan_exclusive_i64(cnt.as<int64_t>(), p, s);
   LSP_HIP(hipMemcpyAsync(cnt.as<int64_t>() + p, &d->num_offsets, sizeof(int64_t), hipMemcpyHostToDevice, s));
   {
