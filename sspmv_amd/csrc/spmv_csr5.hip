@@ -204,12 +204,16 @@ __global__ void scale_list(int64_t n, const int* __restrict__ list, V* __restric
 
 // ---- SpMV -------------------------------------------------------------------
 
-// The strided instantiations (S, SpMM one column at a time) keep some of the
-// 32-slot loops rolled (the compiler reports it); the SpMV ones unroll fully.
+// SIG > 0: sigma fixed at compile time (8 or 16), so the lane's SIG column /
+// value slots and the loops over them unroll into registers. SIG = 0: sigma
+// known at run time only (any other value in 1..32); the compiler then keeps
+// the 32-slot arrays in scratch memory (272 bytes a lane: 1.1 GB written per
+// launch on the lognormal matrix before the specialisations) and reports
+// loops it leaves rolled.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Wpass-failed"
-template <class V, int W, bool S>
-__global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int bit_y, int bit_all,
+template <class V, int W, bool S, int SIG>
+__global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma_rt, int bit_y, int bit_all,
                                                      const uint32_t* __restrict__ tp,
                                                      const uint32_t* __restrict__ desc,
                                                      const int* __restrict__ dop, const int* __restrict__ dof,
@@ -220,6 +224,8 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
   const int l = threadIdx.x & (W - 1);
   const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / W;
   if (t >= p - 1) return;  // whole W-lane groups leave together
+  constexpr int NS = SIG > 0 ? SIG : 32;  // slots of the register arrays
+  const int sigma = SIG > 0 ? SIG : sigma_rt;
   const int64_t tile = (int64_t)W * sigma;
   const uint32_t raw = tp[t];
   const int rs = (int)(raw & kMask), re = (int)(tp[t + 1] & kMask);
@@ -228,22 +234,22 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
   const int* tc = col + t * tile;
   const V* tv = val + t * tile;
   // all loads of the lane issued before use
-  int c[32];
-  V v[32];
+  int c[NS];
+  V v[NS];
 #pragma unroll
-  for (int i = 0; i < 32; i++) {
+  for (int i = 0; i < NS; i++) {
     if (i < sigma) {
       c[i] = tc[(int64_t)i * W + l];
       v[i] = tv[(int64_t)i * W + l];
     }
   }
 #pragma unroll
-  for (int i = 0; i < 32; i++)
+  for (int i = 0; i < NS; i++)
     if (i < sigma) v[i] *= x[at<S>(c[i], ldx)];
   if (rs == re) {  // the whole tile lies inside one row
     V s = 0;
 #pragma unroll
-    for (int i = 0; i < 32; i++)
+    for (int i = 0; i < NS; i++)
       if (i < sigma) s += v[i];
 #pragma unroll
     for (int o = W / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, W);
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma, int b
   V acc = 0, head = 0;
   int seg = -1;  // open segment started in this lane (-1: the inherited head is open)
 #pragma unroll
-  for (int i = 0; i < 32; i++) {
+  for (int i = 0; i < NS; i++) {
     if (i >= sigma) break;
     if ((flags >> (31 - i)) & 1) {
       if (seg < 0) {
@@ -353,15 +359,26 @@ __global__ void csr5_tail(int64_t m, int64_t tail_start, int64_t first_tail, con
   else *yr = beta == V(0) ? alpha * sum : alpha * sum + beta * *yr;
 }
 
-template <class V, int W, bool S>
-void launch_tiles(Matrix& A, Csr5Data& d, double alpha, const void* x, double beta, void* y, int64_t ldx,
-                  int64_t ldy, hipStream_t s) {
+template <class V, int W, bool S, int SIG>
+void launch_tiles_sig(Csr5Data& d, double alpha, const void* x, double beta, void* y, int64_t ldx, int64_t ldy,
+                      hipStream_t s) {
   const int bit_all = d.bit_y + d.bit_scan;
-  hipLaunchKernelGGL((csr5_tiles<V, W, S>), grid1d((d.p - 1) * W, kBlock), dim3(kBlock), 0, s, d.p, d.sigma,
+  hipLaunchKernelGGL((csr5_tiles<V, W, S, SIG>), grid1d((d.p - 1) * W, kBlock), dim3(kBlock), 0, s, d.p, d.sigma,
                      d.bit_y, bit_all, d.tile_ptr.as<uint32_t>(), d.tile_desc.as<uint32_t>(),
                      d.desc_off_ptr.as<int>(), d.desc_off.as<int>(), d.col.as<int>(), d.val.as<V>(),
                      static_cast<const V*>(x), static_cast<V*>(y), d.calib.as<V>(), (V)alpha, (V)beta, ldx, ldy);
   LSP_LAUNCHED();
+}
+
+// Sigma 8 or 16 (the reference's CSR5_SIGMA and the tuning grid's values)
+// takes the register-resident instantiations.
+template <class V, int W, bool S>
+void launch_tiles(Matrix& A, Csr5Data& d, double alpha, const void* x, double beta, void* y, int64_t ldx,
+                  int64_t ldy, hipStream_t s) {
+  (void)A;
+  if (d.sigma == 16) return launch_tiles_sig<V, W, S, 16>(d, alpha, x, beta, y, ldx, ldy, s);
+  if (d.sigma == 8) return launch_tiles_sig<V, W, S, 8>(d, alpha, x, beta, y, ldx, ldy, s);
+  launch_tiles_sig<V, W, S, 0>(d, alpha, x, beta, y, ldx, ldy, s);
 }
 
 // One product: y = alpha A x + beta y, with x and y strided by ldx / ldy
