@@ -207,6 +207,7 @@ void csr_release_aux(Matrix& A);
 // tiles with their carry fix-up (carry_row int32[tiles], carry_val and
 // head_val V[tiles]).
 constexpr int kMergeItems = 1024;
+constexpr int kMergeThreads = 256;  // threads of a merge-path tile's workgroup
 void csr_merge_partition(int64_t tiles, int64_t m, int64_t nnz, const int* rp, int* coord, hipStream_t s);
 void csr_merge_tiles(int dtype, int64_t m, int64_t tiles, const int* rp, const int* ci, const void* val,
                      const void* x, void* y, double alpha, double beta, const int* coord, int* carry_row,

@@ -23,13 +23,17 @@ namespace {
 
 constexpr int kBlock = 256;
 // merge-path items (row ends + entries) per tile of the CSR kernel:
-// kMergeItems = 1024 (stencil 369 against 374 us with 2048, FP32 lognormal 692
-// against 699)
-// items per tile of the column-panel kernel: 1024 for FP64 (555 against
-// 566 us on the lognormal matrix: more workgroups fit), 2048 for FP32 (431
-// against 449 us with 1024)
+// kMergeItems = 1024 on 256 threads (stencil 369 against 374 us with 2048,
+// FP32 lognormal 692 against 699; 2048 items on 512 threads: stencil 367
+// against 363, FP32 702 against 692, FP64 lognormal 841 against 842)
+// items per tile of the column-panel kernel and threads of its workgroup:
+// FP64 2048 items on 512 threads (four entries per thread; 1024 items on 256
+// threads gave 555 against 566 us for 2048 on 256, whose LDS let fewer
+// workgroups fit), FP32 2048 items on 256 threads (431 against 449 us with 1024)
 template <class V>
-constexpr int kPanelItems = sizeof(V) == 8 ? 1024 : 2048;
+constexpr int kPanelItems = 2048;
+template <class V>
+constexpr int kPanelThreads = sizeof(V) == 8 ? 512 : 256;
 
 template <int L, class V>
 __device__ __forceinline__ V group_sum(V v) {
@@ -98,7 +102,7 @@ __global__ void merge_partition_kernel(int64_t tiles, int64_t m, int64_t nnz, co
   coord[2 * t + 1] = (int)(d - r);
 }
 
-template <class V>
+template <class V, int BT = kBlock>
 __device__ V block_sum(V v, V* red) {
   v = group_sum<64>(v);
   const int w = threadIdx.x >> 6;
@@ -106,7 +110,7 @@ __device__ V block_sum(V v, V* red) {
   __syncthreads();
   V t = 0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < kBlock / 64; i++) t += red[i];
+    for (int i = 0; i < BT / 64; i++) t += red[i];
   return t;
 }
 
@@ -127,7 +131,7 @@ __device__ __forceinline__ T stream_load(const T* p) {
 // against four bytes per row. With COMPACT the column of entry e (an index
 // into the panel run, x already offset to the panel's first column) is
 // ci16[e] | (nibble e of cihi) << 16: 2.5 bytes per entry instead of 4.
-template <class V, bool NT, int ITEMS, bool BITS = false, bool COMPACT = false>
+template <class V, bool NT, int ITEMS, bool BITS = false, bool COMPACT = false, int BT = kBlock>
 __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restrict__ rp,
                                            const int* __restrict__ ci, const V* __restrict__ val,
                                            const V* __restrict__ x, V* __restrict__ y, V alpha, V beta,
@@ -136,10 +140,10 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
                                            const uint32_t* __restrict__ ends = nullptr,
                                            const uint16_t* __restrict__ ci16 = nullptr,
                                            const uint8_t* __restrict__ cihi = nullptr, int64_t off = 0) {
-  constexpr int Q = ITEMS / kBlock;  // entries staged per thread
+  constexpr int Q = ITEMS / BT;  // entries staged per thread (BT threads in the workgroup)
   __shared__ V prod[ITEMS];
   __shared__ int rows[ITEMS + 2];
-  __shared__ V red[kBlock / 64];
+  __shared__ V red[BT / 64];
   __shared__ int long_rows[ITEMS / kLongRow + 2];
   __shared__ int nlong;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -153,7 +157,7 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
   V v[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    const int i = tid + q * kBlock;
+    const int i = tid + q * BT;
     const bool ok = i < nz;
     if constexpr (COMPACT) {
       const int64_t e = off + z0 + i;
@@ -189,14 +193,14 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
       if (lane == 0) rows[0] = stream_load<NT>(rp + r0);
     }
   } else {
-    for (int i = tid; i <= last - r0; i += kBlock) rows[i] = stream_load<NT>(rp + r0 + i);
+    for (int i = tid; i <= last - r0; i += BT) rows[i] = stream_load<NT>(rp + r0 + i);
   }
   V xv[Q];
 #pragma unroll
-  for (int q = 0; q < Q; q++) xv[q] = (tid + q * kBlock < nz) ? x[cidx[q]] : V(0);
+  for (int q = 0; q < Q; q++) xv[q] = (tid + q * BT < nz) ? x[cidx[q]] : V(0);
 #pragma unroll
   for (int q = 0; q < Q; q++) {
-    const int i = tid + q * kBlock;
+    const int i = tid + q * BT;
     if (i < nz) prod[i] = v[q] * xv[q];
   }
   __syncthreads();
@@ -207,7 +211,7 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
   // is uniform, so every lane takes part in the shuffles.
   const int nrows = r1 - r0;
   const int rl = nz >= 16 * nrows ? 4 : 1;
-  for (int base = 0; base < nrows; base += kBlock / rl) {
+  for (int base = 0; base < nrows; base += BT / rl) {
     const int i = base + tid / rl, q = tid % rl;
     V sum = 0;
     bool mine = false;
@@ -232,7 +236,7 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
   }
   __syncthreads();
   // long rows: one wavefront each
-  for (int q = wave; q < nlong; q += kBlock / 64) {
+  for (int q = wave; q < nlong; q += BT / 64) {
     const int i = long_rows[q];
     const int s = rows[i], e = rows[i + 1];
     const int sl = (s > z0 ? s : z0) - z0, el = e - z0;
@@ -250,8 +254,8 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
     const int s = rows[r1 - r0];
     const int sl = (s > z0 ? s : z0) - z0;
     V part = 0;
-    for (int k = sl + tid; k < nz; k += kBlock) part += prod[k];
-    part = block_sum(part, red);
+    for (int k = sl + tid; k < nz; k += BT) part += prod[k];
+    part = block_sum<V, BT>(part, red);
     if (tid == 0) {
       carry_row[b] = r1;
       carry_val[b] = part;
@@ -262,13 +266,14 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
 }
 
 template <class V>
-__global__ __launch_bounds__(kBlock) void csr_merge_kernel(int64_t m, const int* __restrict__ rp,
+__global__ __launch_bounds__(kMergeThreads) void csr_merge_kernel(int64_t m, const int* __restrict__ rp,
                                                            const int* __restrict__ ci, const V* __restrict__ val,
                                                            const V* __restrict__ x, V* __restrict__ y, V alpha,
                                                            V beta, const int* __restrict__ coord,
                                                            int* __restrict__ carry_row, V* __restrict__ carry_val,
                                                            V* __restrict__ head_val) {
-  merge_tile<V, false, kMergeItems>((int)xcd_block(blockIdx.x, gridDim.x), m, rp, ci, val, x, y, alpha, beta, coord,
+  merge_tile<V, false, kMergeItems, false, false, kMergeThreads>((int)xcd_block(blockIdx.x, gridDim.x), m, rp, ci,
+                                                                 val, x, y, alpha, beta, coord,
                                     carry_row, carry_val, head_val);
 }
 
@@ -298,7 +303,7 @@ struct PanelArgs {
 };
 
 template <class V, bool COMPACT>
-__global__ __launch_bounds__(kBlock) void panel_merge_kernel(int64_t m, PanelArgs a, const int* __restrict__ ci,
+__global__ __launch_bounds__(kPanelThreads<V>) void panel_merge_kernel(int64_t m, PanelArgs a, const int* __restrict__ ci,
                                                              const V* __restrict__ val, const V* __restrict__ x,
                                                              V* __restrict__ partial, V* __restrict__ carry_val,
                                                              V* __restrict__ head_val) {
@@ -314,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void panel_merge_kernel(int64_t m, PanelArg
   // (a panel past the last column has no entries; its base is clamped to n)
   const int64_t first = (int64_t)p * a.width < a.n ? (int64_t)p * a.width : a.n;
   const V* xs = COMPACT ? x + first : x;
-  merge_tile<V, true, kPanelItems<V>, true, COMPACT>(
+  merge_tile<V, true, kPanelItems<V>, true, COMPACT, kPanelThreads<V>>(
       (int)t, m, a.prp + (int64_t)p * (m + 1), COMPACT ? nullptr : ci + off, val + off, xs, partial + (int64_t)p * m,
       V(1), V(0),
       a.coord + (int64_t)p * 2 * (a.maxT + 1), a.carry_row + (int64_t)p * a.maxT, carry_val + (int64_t)p * a.maxT,
@@ -557,7 +562,7 @@ void merge_launch(int64_t m, int64_t tiles, const int* rp, const int* ci, const 
                   double alpha, double beta, const int* coord, int* carry_row, V* carry_val, V* head_val,
                   hipStream_t s) {
   if (tiles == 0) return;
-  hipLaunchKernelGGL((csr_merge_kernel<V>), dim3((unsigned)tiles), dim3(kBlock), 0, s, m, rp, ci, val,
+  hipLaunchKernelGGL((csr_merge_kernel<V>), dim3((unsigned)tiles), dim3(kMergeThreads), 0, s, m, rp, ci, val,
                      static_cast<const V*>(x), static_cast<V*>(y), (V)alpha, (V)beta, coord, carry_row, carry_val,
                      head_val);
   LSP_LAUNCHED();
@@ -677,7 +682,8 @@ void panel_run(Matrix& A, int P, double alpha, const void* x, double beta, void*
               mt.p_cihi.as<uint8_t>()};
   const int64_t blocks = share > 1 ? kXcds * ceil_div(mt.p_maxT, share) : P * mt.p_maxT;
   auto kern = mt.p_compact ? panel_merge_kernel<V, true> : panel_merge_kernel<V, false>;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kBlock), 0, s, A.m, a, mt.p_ci.as<int>(), mt.p_val.as<V>(),
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kPanelThreads<V>), 0, s, A.m, a, mt.p_ci.as<int>(),
+                     mt.p_val.as<V>(),
                      static_cast<const V*>(x), mt.p_partial.as<V>(), mt.p_carry_val.as<V>(), mt.p_head.as<V>());
   LSP_LAUNCHED();
   hipLaunchKernelGGL((panel_fixup_kernel<V>), grid1d(P * mt.p_maxT, kBlock), dim3(kBlock), 0, s, A.m, a,
