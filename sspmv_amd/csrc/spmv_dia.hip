@@ -26,16 +26,38 @@ constexpr int kMaxClusters = 16;
 constexpr int kLdsBytes = 48 * 1024;
 
 // first[m + c - i] = the smallest entry index on diagonal c - i (its first
-// appearance in row-major order). The stored minimum only decreases, so a
-// plain read that is already <= j (possibly an older, larger value: then the
-// atomic runs anyway) proves the atomic useless: on a banded matrix nearly all
-// of them are skipped (27-point stencil: 109M atomics on 27 addresses before).
-__global__ void diag_first(int64_t m, const int* __restrict__ rp, const int* __restrict__ ci, int* first) {
+// appearance in row-major order). Lane i takes row i; at step q every lane
+// holds the q-th entry of its row, and entry indices grow with the lane. The
+// lanes agreeing on a diagonal are served by one atomic from the lowest of
+// them (the smallest index), so a banded matrix costs one atomic per diagonal
+// per wave and step; and as the stored minimum only decreases, a plain read
+// that is already <= that index (possibly an older, larger value: then the
+// atomic runs anyway) skips it. (27-point stencil: 109M atomics on 27
+// addresses before.)
+__global__ __launch_bounds__(kBlock) void diag_first(int64_t m, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci, int* first) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  for (int j = rp[i]; j < rp[i + 1]; j++) {
-    int* f = first + (m - i) + ci[j];
-    if (*f > j) atomicMin(f, j);
+  const bool row_ok = i < m;
+  const int s = row_ok ? rp[i] : 0, e = row_ok ? rp[i + 1] : 0;
+  int len = e - s;
+  // the wave's longest row bounds the steps (every lane takes part in the ballots)
+  for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off));
+  const int lane = threadIdx.x & 63;
+  for (int q = 0; q < len; q++) {
+    const int j = s + q;
+    const bool has = j < e;
+    const int64_t k = has ? (m - i) + ci[j] : -1;
+    uint64_t todo = __ballot(has);
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const long long kl = __shfl((long long)k, leader);
+      const uint64_t same = __ballot(has && k == kl) & todo;
+      if (lane == leader) {
+        int* f = first + kl;
+        if (*f > j) atomicMin(f, j);
+      }
+      todo &= ~same;
+    }
   }
 }
 
