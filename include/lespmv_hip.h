@@ -435,7 +435,7 @@ int lespmv_block_counts(lespmv_matrix_t A, const int32_t* dims, int32_t n, int64
  * DIA variant, BSR block, CSR5 (omega, sigma), panels (8, 4); a class's
  * parameter head indexes its run [class_first[c], class_first[c+1]) of the
  * standard tuning grid (all formats and LESPMV_TUNE_PANELS).
- * allowed[32] marks what the features already rule out (ELL / DIA fill,
+ * allowed[35] marks what the features already rule out (ELL / DIA fill,
  * diagonal count, BSR fill of the counted block sizes). */
 #define LESPMV_SELECTOR_INPUTS 128
 #define LESPMV_SELECTOR_OUTPUTS 35
