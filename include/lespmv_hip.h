@@ -75,9 +75,12 @@ enum {
 
 /* Kernel variants (the GPU meaning of the reference's kernel_flag/kernel_tag). */
 enum {
-  LESPMV_VARIANT_AUTO = 0,      /* the handle's tuned choice; CSR: PANEL when x spans at least 16 MB and a
-                                   third of the entries lie more than 2 MB of x off the diagonal, MERGE when the
-                                   longest row exceeds 32 x max(mean, 4), VECTOR otherwise */
+  LESPMV_VARIANT_AUTO = 0,      /* the handle's tuned choice; CSR: PANEL when x spans at least 16 MB and
+                                   there are 0.3 far x lines per entry or more (entries more than 2 MB of x off
+                                   the diagonal that open a new 128-byte line of x in their row); MERGE when the
+                                   longest row exceeds 32 x max(mean, 4), or when the matrix exceeds 256 MB
+                                   (the last-level cache) and its rows average at most 32 entries or the
+                                   longest exceeds 8 x the mean; VECTOR otherwise (lespmv_csr_plan) */
   LESPMV_VARIANT_VECTOR = 1,    /* CSR: lanes-per-row vector kernel; DIA: x through caches */
   LESPMV_VARIANT_MERGE = 2,     /* CSR: merge-path tiles; DIA: LDS-staged x windows */
   LESPMV_VARIANT_ATOMIC = 3,    /* COO: atomics instead of the carry fix-up pass */
@@ -154,6 +157,11 @@ int lespmv_has_format(lespmv_matrix_t A, int fmt);
  * Synchronous; lespmv_convert_ms(A, LESPMV_CSR) then gives its device time. A
  * PANEL call without it builds the layout on its first call (synchronously). */
 int lespmv_prepare(lespmv_matrix_t A, int fmt, int variant);
+/* What CSR variant AUTO runs for this handle, the vector kernel's lanes per
+ * row, and the far x lines per entry the choice was made from (runs the row
+ * analysis if no call has yet; synchronous). No reference counterpart: the
+ * reference's LeSpMV_csr runs the kernel_flag it is given. */
+int lespmv_csr_plan(lespmv_matrix_t A, int32_t* variant, int32_t* lanes, double* far_lines);
 /* Device milliseconds of the last lespmv_convert of `fmt` (hipEvent timed);
  * for LESPMV_CSR the last lespmv_prepare. */
 int lespmv_convert_ms(lespmv_matrix_t A, int fmt, double* ms);

@@ -399,6 +399,19 @@ int lespmv_prepare(lespmv_matrix_t h, int fmt, int variant) {
   });
 }
 
+int lespmv_csr_plan(lespmv_matrix_t h, int32_t* variant, int32_t* lanes, double* far_lines) {
+  return guarded([&] {
+    Matrix& A = mat(h);
+    if (!A.meta.analysed) {
+      if (A.last_stream && A.last_stream != A.stream) LSP_HIP(hipEventSynchronize(A.ev_switch));
+      csr_analyse(A, A.stream);
+    }
+    if (variant) *variant = A.meta.auto_variant;
+    if (lanes) *lanes = A.meta.lanes;
+    if (far_lines) *far_lines = A.meta.far_frac;
+  });
+}
+
 int lespmv_release_format(lespmv_matrix_t h, int fmt) {
   return guarded([&] {
     Matrix& A = mat(h);

@@ -161,7 +161,7 @@ struct CsrMeta {
   double avg_row = 0;
   int32_t lanes = 0;         // tuned lanes-per-row of the vector kernel
   int32_t auto_variant = 1;  // tuned CSR kernel
-  double far_frac = 0;       // share of entries more than 2 MB of x away from the diagonal
+  double far_frac = 0;       // far x lines per entry (entries > 2 MB of x off the diagonal opening a new line)
   int64_t merge_tiles = 0;
   DBuf merge_coord;          // int32[2*(tiles+1)] (row, nnz) at each tile start
   DBuf carry_row;            // int32[tiles]  row open at the tile end (-1: none)
@@ -209,7 +209,7 @@ void csr_release_aux(Matrix& A);
 constexpr int kMergeItems = 1024;
 constexpr int kMergeThreads = 256;  // threads of a merge-path tile's workgroup
 void csr_merge_partition(int64_t tiles, int64_t m, int64_t nnz, const int* rp, int* coord, hipStream_t s);
-void csr_merge_tiles(int dtype, int64_t m, int64_t tiles, const int* rp, const int* ci, const void* val,
+void csr_merge_tiles(int dtype, int64_t m, int64_t n, int64_t tiles, const int* rp, const int* ci, const void* val,
                      const void* x, void* y, double alpha, double beta, const int* coord, int* carry_row,
                      void* carry_val, void* head_val, hipStream_t s);
 // Device address of the allocation-free SpMV's fixed scratch on `device`

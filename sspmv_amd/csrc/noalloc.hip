@@ -85,7 +85,7 @@ int lespmv_csr_spmv_noalloc(int dtype, int64_t m, int64_t n, int64_t nnz, const 
     void* head_val = slots + sizeof(double) * kTiles;
     hipStream_t s = static_cast<hipStream_t>(stream);
     csr_merge_partition(tiles, m, nnz, row_offset, coord, s);
-    csr_merge_tiles(dtype, m, tiles, row_offset, col_index, values, x, y, alpha, beta, coord, carry_row, carry_val,
+    csr_merge_tiles(dtype, m, n, tiles, row_offset, col_index, values, x, y, alpha, beta, coord, carry_row, carry_val,
                     head_val, s);
   });
 }

@@ -15,6 +15,7 @@
 // rows_engine.hpp (variants 16 + log2 L).
 #include <algorithm>
 
+#include "buffer_loads.hpp"
 #include "common.hpp"
 #include "rows_engine.hpp"
 
@@ -131,6 +132,8 @@ __device__ __forceinline__ T stream_load(const T* p) {
 // against four bytes per row. With COMPACT the column of entry e (an index
 // into the panel run, x already offset to the panel's first column) is
 // ci16[e] | (nibble e of cihi) << 16: 2.5 bytes per entry instead of 4.
+// When x's extent xbytes (bytes from x to the last column any entry names)
+// is below 2^32, x is gathered through a buffer resource.
 template <class V, bool NT, int ITEMS, bool BITS = false, bool COMPACT = false, int BT = kBlock>
 __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restrict__ rp,
                                            const int* __restrict__ ci, const V* __restrict__ val,
@@ -139,7 +142,8 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
                                            V* __restrict__ carry_val, V* __restrict__ head_val,
                                            const uint32_t* __restrict__ ends = nullptr,
                                            const uint16_t* __restrict__ ci16 = nullptr,
-                                           const uint8_t* __restrict__ cihi = nullptr, int64_t off = 0) {
+                                           const uint8_t* __restrict__ cihi = nullptr, int64_t off = 0,
+                                           int64_t xbytes = 0) {
   constexpr int Q = ITEMS / BT;  // entries staged per thread (BT threads in the workgroup)
   __shared__ V prod[ITEMS];
   __shared__ int rows[ITEMS + 2];
@@ -196,8 +200,14 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
     for (int i = tid; i <= last - r0; i += BT) rows[i] = stream_load<NT>(rp + r0 + i);
   }
   V xv[Q];
+  if (xbytes > 0 && xbytes <= kMaxBufferBytes) {  // uniform over the workgroup
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(x, xbytes);
 #pragma unroll
-  for (int q = 0; q < Q; q++) xv[q] = (tid + q * BT < nz) ? x[cidx[q]] : V(0);
+    for (int q = 0; q < Q; q++) xv[q] = (tid + q * BT < nz) ? buf_at<V>(xr, cidx[q]) : V(0);
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; q++) xv[q] = (tid + q * BT < nz) ? x[cidx[q]] : V(0);
+  }
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const int i = tid + q * BT;
@@ -266,15 +276,15 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
 }
 
 template <class V>
-__global__ __launch_bounds__(kMergeThreads) void csr_merge_kernel(int64_t m, const int* __restrict__ rp,
+__global__ __launch_bounds__(kMergeThreads) void csr_merge_kernel(int64_t m, int64_t n, const int* __restrict__ rp,
                                                            const int* __restrict__ ci, const V* __restrict__ val,
                                                            const V* __restrict__ x, V* __restrict__ y, V alpha,
                                                            V beta, const int* __restrict__ coord,
                                                            int* __restrict__ carry_row, V* __restrict__ carry_val,
                                                            V* __restrict__ head_val) {
-  merge_tile<V, false, kMergeItems, false, false, kMergeThreads>((int)xcd_block(blockIdx.x, gridDim.x), m, rp, ci,
-                                                                 val, x, y, alpha, beta, coord,
-                                    carry_row, carry_val, head_val);
+  merge_tile<V, false, kMergeItems, false, false, kMergeThreads>(
+      (int)xcd_block(blockIdx.x, gridDim.x), m, rp, ci, val, x, y, alpha, beta, coord, carry_row, carry_val, head_val,
+      nullptr, nullptr, nullptr, 0, n * (int64_t)sizeof(V));
 }
 
 // Column panels: the matrix split into P column ranges (P a multiple of the 8
@@ -323,7 +333,8 @@ __global__ __launch_bounds__(kPanelThreads<V>) void panel_merge_kernel(int64_t m
       (int)t, m, a.prp + (int64_t)p * (m + 1), COMPACT ? nullptr : ci + off, val + off, xs, partial + (int64_t)p * m,
       V(1), V(0),
       a.coord + (int64_t)p * 2 * (a.maxT + 1), a.carry_row + (int64_t)p * a.maxT, carry_val + (int64_t)p * a.maxT,
-      head_val + (int64_t)p * a.maxT, a.ends + a.endoff[p], a.ci16, a.cihi, off);
+      head_val + (int64_t)p * a.maxT, a.ends + a.endoff[p], a.ci16, a.cihi, off,
+      (COMPACT ? (a.n - first < a.width ? a.n - first : a.width) : a.n) * (int64_t)sizeof(V));
 }
 
 template <class V>
@@ -558,11 +569,12 @@ void vector_dispatch(Matrix& A, int lanes, double alpha, const void* x, double b
 // The merge-path tiles and their carry fix-up over a partition `coord` of
 // `tiles` tiles; carry_row / carry_val / head_val hold one entry per tile.
 template <class V>
-void merge_launch(int64_t m, int64_t tiles, const int* rp, const int* ci, const V* val, const void* x, void* y,
+void merge_launch(int64_t m, int64_t n, int64_t tiles, const int* rp, const int* ci, const V* val, const void* x,
+                  void* y,
                   double alpha, double beta, const int* coord, int* carry_row, V* carry_val, V* head_val,
                   hipStream_t s) {
   if (tiles == 0) return;
-  hipLaunchKernelGGL((csr_merge_kernel<V>), dim3((unsigned)tiles), dim3(kMergeThreads), 0, s, m, rp, ci, val,
+  hipLaunchKernelGGL((csr_merge_kernel<V>), dim3((unsigned)tiles), dim3(kMergeThreads), 0, s, m, n, rp, ci, val,
                      static_cast<const V*>(x), static_cast<V*>(y), (V)alpha, (V)beta, coord, carry_row, carry_val,
                      head_val);
   LSP_LAUNCHED();
@@ -574,7 +586,7 @@ void merge_launch(int64_t m, int64_t tiles, const int* rp, const int* ci, const 
 template <class V>
 void merge_run(Matrix& A, double alpha, const void* x, double beta, void* y, hipStream_t s) {
   CsrMeta& mt = A.meta;
-  merge_launch<V>(A.m, mt.merge_tiles, A.rp.as<int>(), A.ci.as<int>(), A.val.as<V>(), x, y, alpha, beta,
+  merge_launch<V>(A.m, A.n, mt.merge_tiles, A.rp.as<int>(), A.ci.as<int>(), A.val.as<V>(), x, y, alpha, beta,
                   mt.merge_coord.as<int>(), mt.carry_row.as<int>(), mt.carry_val.as<V>(), mt.head_val.as<V>(), s);
 }
 
@@ -764,16 +776,20 @@ void spmm_dispatch(Matrix& A, int k, double alpha, const void* X, int64_t ldx, d
 
 // Entries whose column lies more than `window` columns from the row's
 // diagonal position r * n / m (counted once, for the kernel choice).
-__global__ __launch_bounds__(kBlock) void far_count_kernel(int64_t m, int64_t n, int64_t window,
+__global__ __launch_bounds__(kBlock) void far_count_kernel(int64_t m, int64_t n, int64_t window, int line_shift,
                                                            const int* __restrict__ rp, const int* __restrict__ ci,
                                                            unsigned long long* __restrict__ out) {
   __shared__ unsigned long long red[kBlock / 64];
   unsigned long long c = 0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < m; r += (int64_t)gridDim.x * blockDim.x) {
     const int64_t d = r * n / m;
+    int prev = -1;  // x line of the row's previous entry
     for (int j = rp[r]; j < rp[r + 1]; j++) {
-      const int64_t dc = ci[j] - d;
-      c += (dc > window || dc < -window) ? 1ull : 0ull;
+      const int col = ci[j];
+      const int64_t dc = col - d;
+      const int line = col >> line_shift;
+      c += ((dc > window || dc < -window) && line != prev) ? 1ull : 0ull;
+      prev = line;
     }
   }
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
@@ -810,14 +826,14 @@ void csr_merge_partition(int64_t tiles, int64_t m, int64_t nnz, const int* rp, i
   LSP_LAUNCHED();
 }
 
-void csr_merge_tiles(int dtype, int64_t m, int64_t tiles, const int* rp, const int* ci, const void* val,
+void csr_merge_tiles(int dtype, int64_t m, int64_t n, int64_t tiles, const int* rp, const int* ci, const void* val,
                      const void* x, void* y, double alpha, double beta, const int* coord, int* carry_row,
                      void* carry_val, void* head_val, hipStream_t s) {
   if (dtype == LESPMV_F64)
-    merge_launch<double>(m, tiles, rp, ci, static_cast<const double*>(val), x, y, alpha, beta, coord, carry_row,
+    merge_launch<double>(m, n, tiles, rp, ci, static_cast<const double*>(val), x, y, alpha, beta, coord, carry_row,
                          static_cast<double*>(carry_val), static_cast<double*>(head_val), s);
   else
-    merge_launch<float>(m, tiles, rp, ci, static_cast<const float*>(val), x, y, alpha, beta, coord, carry_row,
+    merge_launch<float>(m, n, tiles, rp, ci, static_cast<const float*>(val), x, y, alpha, beta, coord, carry_row,
                         static_cast<float*>(carry_val), static_cast<float*>(head_val), s);
 }
 
@@ -826,7 +842,9 @@ void csr_analyse(Matrix& A, hipStream_t s) {
   mt.max_row = A.m ? reduce_max_rowlen(A.rp.as<int32_t>(), A.m, s) : 0;
   mt.avg_row = A.m ? (double)A.nnz / (double)A.m : 0.0;
   mt.lanes = lanes_for(mt.avg_row, mt.max_row);
-  // share of entries gathered from x more than 2 MB away from the diagonal
+  // far x lines per entry: entries more than 2 MB of x away from the
+  // diagonal that open a new 128-byte line of x in their row (columns in
+  // runs, as in small dense blocks, share a line and count once)
   mt.far_frac = 0.0;
   if (A.m && A.nnz) {
     DBuf cnt;
@@ -834,26 +852,39 @@ void csr_analyse(Matrix& A, hipStream_t s) {
     LSP_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
     const int64_t window = (int64_t{2} << 20) / (int64_t)vsize(A.dtype);
     hipLaunchKernelGGL(far_count_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(A.m, kBlock), 4096)), dim3(kBlock),
-                       0, s, A.m, A.n, window, A.rp.as<int>(), A.ci.as<int>(), cnt.as<unsigned long long>());
+                       0, s, A.m, A.n, window, vsize(A.dtype) == 8 ? 4 : 5, A.rp.as<int>(), A.ci.as<int>(),
+                       cnt.as<unsigned long long>());
     LSP_LAUNCHED();
     unsigned long long far = 0;
     LSP_HIP(hipMemcpyAsync(&far, cnt.p, sizeof(far), hipMemcpyDeviceToHost, s));
     LSP_HIP(hipStreamSynchronize(s));
     mt.far_frac = (double)far / (double)A.nnz;
   }
-  // Column panels when x is at least four times an XCD's 4 MB L2 and a third
-  // of the entries gather from far off the diagonal: every XCD then keeps its
-  // own slice of x (lognormal matrix of 4M columns, 70 % uniform columns:
-  // FP64 531 us against 821 us for the vector kernel, FP32 396 against 694).
-  // Otherwise the merge path, which bounds every workgroup's work, when the
-  // longest row is far beyond the mean (timings on the selector's synthetic
-  // families favour the vector kernel below that).
-  const double xbytes = (double)A.n * (double)vsize(A.dtype);
+  // Column panels when x is at least four times an XCD's 4 MB L2 and there
+  // are at least 0.3 far x lines per entry: every XCD then keeps its own slice
+  // of x (lognormal matrix of 4M columns, 70 % uniform columns: FP64 495 us
+  // against 830 us for the vector kernel, FP32 378 against 697; random 4x4
+  // blocks, a quarter line per entry, run faster without). Otherwise the
+  // merge path, which bounds every workgroup's work, when the longest row is
+  // far beyond the mean, and also for a matrix that does not fit the 256 MB
+  // last-level cache whose rows are short or long-tailed: read from HBM,
+  // the merge tiles' staged loads win there (27-point stencils of 37M-109M
+  // entries 0.88-0.90x the vector kernel's time, banded and block-diagonal
+  // rows of 13-30 entries 0.76-0.98x; rows of 50 entries and more 1.1-1.3x),
+  // while on cache-resident matrices the vector kernel is faster (2-D
+  // Laplacians of 7M-14M entries 1.12-1.23x for the merge path, and the
+  // selector's synthetic families).
+  const double sv = (double)vsize(A.dtype);
+  const double xbytes = (double)A.n * sv;
+  const double bytes = (double)A.nnz * (4.0 + sv) + 4.0 * (double)(A.m + 1) + sv * (double)(A.n + A.m);
   if (mt.far_frac >= 0.3 && xbytes >= 16.0 * (1 << 20))
     mt.auto_variant = LESPMV_VARIANT_PANEL;
+  else if (mt.max_row > 32 * std::max<double>(mt.avg_row, 4.0))
+    mt.auto_variant = LESPMV_VARIANT_MERGE;
+  else if (bytes > 256.0 * (1 << 20) && (mt.avg_row <= 32.0 || mt.max_row > 8.0 * mt.avg_row))
+    mt.auto_variant = LESPMV_VARIANT_MERGE;
   else
-    mt.auto_variant = mt.max_row > 32 * std::max<double>(mt.avg_row, 4.0) ? LESPMV_VARIANT_MERGE
-                                                                          : LESPMV_VARIANT_VECTOR;
+    mt.auto_variant = LESPMV_VARIANT_VECTOR;
   const int64_t items = A.m + A.nnz;
   mt.merge_tiles = ceil_div(items, kMergeItems);
   mt.merge_coord.alloc(sizeof(int) * 2 * (mt.merge_tiles + 1));

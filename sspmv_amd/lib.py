@@ -126,6 +126,7 @@ def load() -> C.CDLL:
         "lespmv_has_format": ([vp, C.c_int], C.c_int),
         "lespmv_convert_ms": ([vp, C.c_int, C.POINTER(dbl)], C.c_int),
         "lespmv_prepare": ([vp, C.c_int, C.c_int], C.c_int),
+        "lespmv_csr_plan": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)], C.c_int),
         "lespmv_csr_spmv_noalloc": ([C.c_int, i64, i64, i64, vp, vp, vp, dbl, vp, dbl, vp, vp], C.c_int),
         "lespmv_csr_noalloc_max_items": ([], i64),
         "lespmv_spmv": ([vp, C.c_int, C.c_int, dbl, vp, dbl, vp, vp], C.c_int),
@@ -378,6 +379,12 @@ class Matrix:
         partition, the column-panel layout); returns its device ms."""
         check(load().lespmv_prepare(self.h, fmt, variant))
         return self.convert_ms(fmt)
+
+    def csr_plan(self):
+        """(variant AUTO runs for CSR, vector lanes per row, far x lines per entry)."""
+        v, lanes, far = C.c_int32(), C.c_int32(), C.c_double()
+        check(load().lespmv_csr_plan(self.h, C.byref(v), C.byref(lanes), C.byref(far)))
+        return v.value, lanes.value, far.value
 
     def convert_ms(self, fmt: int) -> float:
         ms = C.c_double()

@@ -334,14 +334,18 @@ def widen_db(db):
     return out
 
 
-def retime_panels(db_path, out, iters=10, max_nnz=6e6, n_max=2e6, rows=None, log=print):
-    """Times the two CSR column-panel candidates on the matrices of a timing
-    database collected before they joined the grid (all rows, or the rows
-    [start, stop) given, which alone are written to `out`). The matrices are
-    made again from their stored seeds (collection seed * 1,000,003 + index:
-    the sampler is replayed from that collection's start, and the regenerated
+def retime(db_path, out, columns, iters=10, max_nnz=6e6, n_max=2e6, rows=None, log=print):
+    """Times the grid candidates `columns` (grid indices; N_GRID is the default
+    CSR kernel) again on the matrices of a timing database (all rows, or the
+    rows [start, stop) given, which alone are written to `out`): after a
+    kernel change, or for the two column-panel candidates (27, 28) in a
+    database collected before they joined the grid. The matrices are made
+    again from their stored seeds (collection seed * 1,000,003 + index: the
+    sampler is replayed from that collection's start, and the regenerated
     sizes and family must equal the stored ones), so the other columns stay
     as they were timed."""
+    columns = [int(c) for c in columns]
+    assert all(0 <= c <= N_GRID for c in columns), columns
     db = widen_db(load_db(db_path))
     if rows is not None:
         db = {k: v[rows[0]:rows[1]] for k, v in db.items()}
@@ -362,12 +366,16 @@ def retime_panels(db_path, out, iters=10, max_nnz=6e6, n_max=2e6, rows=None, log
         A = lib.generate(gen, params, int(db["seed"][r]))
         assert (A.m, A.nnz, FAMILIES.index(fam)) == (db["m"][r], db["nnz"][r], db["family"][r]), r
         M = lib.Matrix.from_host(A)
-        cands = (lib.TuneResultC * 2)()
-        cands[0], cands[1] = grid[27], grid[28]
+        cands = (lib.TuneResultC * len(columns))()
+        for j, c in enumerate(columns):
+            if c < N_GRID:
+                cands[j] = grid[c]
+            else:
+                cands[j].fmt, cands[j].variant = lib.CSR, lib.VARIANT_AUTO
         M.tune(cands, iters=iters)
-        for j in range(2):
-            db["times"][r, 27 + j] = cands[j].ms if cands[j].status == 0 else np.inf
-            db["convert_ms"][r, 27 + j] = cands[j].convert_ms
+        for j, c in enumerate(columns):
+            db["times"][r, c] = cands[j].ms if cands[j].status == 0 else np.inf
+            db["convert_ms"][r, c] = cands[j].convert_ms
         del M, A
         if (r + 1) % 500 == 0 or r + 1 == len(seeds):
             log(f"retime {r + 1}/{len(seeds)} {time.time() - t0:.0f}s", flush=True)
@@ -404,12 +412,17 @@ def main(argv=None):
     t.add_argument("--tie", type=float, default=0.0)
     t.add_argument("--weight-decay", type=float, default=1e-4)
     t.add_argument("--hidden", type=int, default=128)
-    rp = sub.add_parser("retime-panels", help="time the CSR column-panel candidates into an older database")
-    rp.add_argument("--db", required=True)
-    rp.add_argument("--out", required=True)
-    rp.add_argument("--iters", type=int, default=10)
-    rp.add_argument("--n-max", type=float, default=2e6)
-    rp.add_argument("--rows", help="START:STOP, the rows to time (written alone to --out)")
+    for name, hlp in (("retime", "time grid candidates again on a database's matrices"),
+                      ("retime-panels", "time the CSR column-panel candidates into an older database")):
+        rp = sub.add_parser(name, help=hlp)
+        rp.add_argument("--db", required=True)
+        rp.add_argument("--out", required=True)
+        rp.add_argument("--iters", type=int, default=10)
+        rp.add_argument("--n-max", type=float, default=2e6)
+        rp.add_argument("--rows", help="START:STOP, the rows to time (written alone to --out)")
+        if name == "retime":
+            rp.add_argument("--columns", required=True,
+                            help=f"comma list of grid indices ({N_GRID} = the default CSR kernel)")
     e = sub.add_parser("evaluate")
     e.add_argument("--db", required=True)
     e.add_argument("--model", required=True)
@@ -418,9 +431,10 @@ def main(argv=None):
     lib.check(lib.load().lespmv_init(0))
     if a.cmd == "collect":
         collect(a.count, a.out, a.seed, a.iters, a.max_nnz, (1e4, a.n_max))
-    elif a.cmd == "retime-panels":
+    elif a.cmd in ("retime", "retime-panels"):
         rows = tuple(int(v) for v in a.rows.split(":")) if a.rows else None
-        retime_panels(a.db, a.out, a.iters, n_max=a.n_max, rows=rows)
+        cols = a.columns.split(",") if a.cmd == "retime" else (N_GRID - 2, N_GRID - 1)
+        retime(a.db, a.out, cols, a.iters, n_max=a.n_max, rows=rows)
     elif a.cmd == "evaluate":
         run_evaluation(a.db, a.model, a.report)
     else:

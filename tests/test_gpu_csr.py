@@ -268,6 +268,7 @@ def test_auto_picks_panels_on_scattered_columns(gpu, oracle):
     grown = M.format_bytes(CSR)
     print(f"prepare (AUTO) {ms:.2f} ms, CSR bytes {base} -> {grown}")
     assert ms > 0 and grown > base + A.nnz * 8  # the panel layout exists
+    assert M.csr_plan()[0] == gpu.VARIANT_PANEL
     x = gpu.fill_uniform(A.n, 4)
     y0 = gpu.fill_uniform(A.m, 9)
     ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
@@ -279,3 +280,29 @@ def test_auto_picks_panels_on_scattered_columns(gpu, oracle):
     MB = gpu.Matrix.from_host(B)
     MB.prepare(CSR, gpu.VARIANT_AUTO)
     assert MB.format_bytes(CSR) < B.nnz * 12 + (B.m + 1) * 4 + 64 * 1024 * 1024  # no panels for a banded matrix
+    assert MB.csr_plan()[0] != gpu.VARIANT_PANEL
+
+
+def test_auto_plan(gpu):
+    """The CSR variant AUTO runs (lespmv_csr_plan): column panels when x spans
+    16 MB or more with 0.3 far x lines per entry or more (random 2x2 blocks:
+    half a line per entry), not for random 4x4 blocks (a quarter line per
+    entry), which at 285 MB and 4 entries a row take the merge path, as does a
+    27-point stencil beyond the 256 MB last-level cache; the vector kernel
+    for a stencil inside it."""
+    cases = [
+        (gpu.GEN_LOGNORMAL, [4194304, 4194304, 2.0, 1.0, 64, 0.3, 64], gpu.VARIANT_PANEL, (0.3, 1.0)),
+        (gpu.GEN_BLOCKRAND, [2097152, 2097152, 2, 2, 4], gpu.VARIANT_PANEL, (0.4, 0.55)),
+        (gpu.GEN_BLOCKRAND, [1048576, 1048576, 4, 4, 1], gpu.VARIANT_MERGE, (0.2, 0.3)),  # 285 MB, 4 per row
+        (gpu.GEN_STENCIL27, [110, 26.0, -1.0], gpu.VARIANT_MERGE, (0.0, 0.01)),
+        (gpu.GEN_STENCIL27, [40, 26.0, -1.0], gpu.VARIANT_VECTOR, (0.0, 0.01)),
+    ]
+    for gen, params, want, (lo, hi) in cases:
+        A = gpu.generate(gen, params, 5)
+        M = gpu.Matrix.from_host(A)
+        variant, lanes, far = M.csr_plan()
+        print(f"generator {gen} {params}: variant {variant}, lanes {lanes}, far lines per entry {far:.3f}")
+        assert variant == want, (gen, params, variant)
+        assert lo <= far < hi, (gen, params, far)
+        assert 2 <= lanes <= 64
+        del M
