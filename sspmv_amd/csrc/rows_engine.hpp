@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "buffer_loads.hpp"
+
 namespace lsp {
 
 constexpr int kItemIters = 64;  // target loads per lane in one work item
@@ -30,6 +32,13 @@ inline int rows_mode(int k, int64_t ldx, int64_t ldy, const void* X, const void*
   const bool vec = k % W == 0 && ldx % W == 0 && ldy % W == 0 && reinterpret_cast<uintptr_t>(X) % align == 0 &&
                    reinterpret_cast<uintptr_t>(Y) % align == 0;
   return W == 2 ? (vec ? 2 : 1) : (vec ? 4 : 3);
+}
+
+// Whether a product's loads fit 32-bit byte offsets from their arrays' bases
+// (entries: nnz * value bytes, X: n rows of ldx), so that the row engine can
+// issue them as buffer loads (run_rows' BUF).
+inline bool rows_buffer_ok(int64_t nnz, int64_t n, int64_t ldx, int64_t vbytes) {
+  return nnz * vbytes <= kMaxBufferBytes && n * ldx * vbytes <= kMaxBufferBytes;
 }
 
 // Rows in one work item: passes of 64 / G rows, enough passes for about
@@ -65,14 +74,40 @@ __device__ __forceinline__ void load_w(const LSP_GLOBAL_AS V* __restrict__ p, in
   }
 }
 
+// The same from byte offset `off` of a buffer resource.
+template <class V, int W, bool VEC>
+__device__ __forceinline__ void load_w_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, int nvalid, V* out) {
+  constexpr int B = W * (int)sizeof(V);
+  if constexpr (VEC && W > 1 && B == 8) {
+    typedef V vec_t __attribute__((ext_vector_type(W)));
+    const vec_t v = __builtin_bit_cast(vec_t, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+#pragma unroll
+    for (int u = 0; u < W; u++) out[u] = v[u];
+  } else if constexpr (VEC && W > 1 && B >= 16) {
+    typedef V half_t __attribute__((ext_vector_type(16 / sizeof(V))));
+    constexpr int H = 16 / (int)sizeof(V);
+#pragma unroll
+    for (int h = 0; h < W / H; h++) {
+      const half_t v = __builtin_bit_cast(half_t, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, h * 16, 0));
+#pragma unroll
+      for (int u = 0; u < H; u++) out[h * H + u] = v[u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < W; u++) out[u] = u < nvalid ? buf_load<V>(r, off, u * (uint32_t)sizeof(V)) : (V)0;
+  }
+}
+
 // Rows [r0, r1) of request q for all k right-hand sides. A row gets G lanes
 // (G = max(lanes, kk), kk = lanes needed to cover k at W right-hand sides
 // each, rounded up to a power of two); lane t handles right-hand sides
 // (t mod kk)*W .. +W-1 and every (G/kk)-th nonzero of the row, and the
 // partial sums meet in xor shuffles. A lane issues the index, value and x
 // loads of kChains nonzeros before it uses any of them (out-of-row slots are
-// masked), so that many gathers are in flight per lane.
-template <class V, int W, bool VEC, class Q>
+// masked), so that many gathers are in flight per lane. With BUF (the product
+// passes rows_buffer_ok) the index, value and X loads are buffer loads: 32-bit
+// offsets, the chain stride in a scalar offset.
+template <class V, int W, bool VEC, bool BUF, class Q>
 __device__ __forceinline__ void run_rows(const Q& q, int r0, int r1, int lane) {
   constexpr int CH = kChains;
   const int k = q.k;
@@ -90,6 +125,12 @@ __device__ __forceinline__ void run_rows(const Q& q, int r0, int r1, int lane) {
   const LSP_GLOBAL_AS int* __restrict__ ci = (const LSP_GLOBAL_AS int*)q.ci;
   const int64_t ldx = q.ldx;
   const double alpha = q.alpha, beta = q.beta;
+  __amdgpu_buffer_rsrc_t cr, vr, xr;
+  if constexpr (BUF) {
+    cr = make_rsrc(q.ci, kMaxBufferBytes);
+    vr = make_rsrc(q.val, kMaxBufferBytes);
+    xr = make_rsrc(q.X, kMaxBufferBytes);
+  }
   for (int base = r0; base < r1; base += P) {
     const int row = base + grp;
     const bool ok = row < r1 && j < k;
@@ -104,13 +145,23 @@ __device__ __forceinline__ void run_rows(const Q& q, int r0, int r1, int lane) {
 #pragma unroll
         for (int c = 0; c < CH; c++) {
           const int pc = p + c * nzl;
-          cc[c] = pc < e ? ci[pc] : 0;
-          vv[c] = pc < e ? val[pc] : (V)0;
+          if constexpr (BUF) {
+            const uint32_t cs = (uint32_t)(c * nzl);  // wave-uniform chain stride
+            cc[c] = pc < e ? buf_load<int>(cr, (uint32_t)p * 4u, cs * 4u) : 0;
+            vv[c] = pc < e ? buf_load<V>(vr, (uint32_t)p * (uint32_t)sizeof(V), cs * (uint32_t)sizeof(V)) : (V)0;
+          } else {
+            cc[c] = pc < e ? ci[pc] : 0;
+            vv[c] = pc < e ? val[pc] : (V)0;
+          }
         }
 #pragma unroll
         for (int c = 0; c < CH; c++) {
           if (p + c * nzl < e) {
-            load_w<V, W, VEC>(X + (int64_t)cc[c] * ldx, nvalid, xx[c]);
+            if constexpr (BUF)
+              load_w_buf<V, W, VEC>(xr, ((uint32_t)cc[c] * (uint32_t)ldx + (uint32_t)j) * (uint32_t)sizeof(V), nvalid,
+                                    xx[c]);
+            else
+              load_w<V, W, VEC>(X + (int64_t)cc[c] * ldx, nvalid, xx[c]);
           } else {
 #pragma unroll
             for (int u = 0; u < W; u++) xx[c][u] = 0;

@@ -50,8 +50,9 @@ struct alignas(16) ReqDesc {
   double alpha, beta;
   int32_t m, k, lanes, rows_per_item;
   int32_t item_begin, nitems;
-  int32_t mode, pad[3];  // right-hand sides per lane and vector access, see rhs_mode
+  int32_t mode, pad[3];  // right-hand sides per lane and vector access (rhs_mode), + kBufferMode for buffer loads
 };
+constexpr int32_t kBufferMode = 8;
 
 // Each request is placed on one XCD: its work items form that XCD's queue
 // [xbegin[x], xbegin[x + 1]) with its own counter, so the request's X is
@@ -115,11 +116,16 @@ __global__ __launch_bounds__(kServeBlock) void packed_kernel(const unsigned char
       const int r0 = (item - s_begin[lo]) * q.rows_per_item;
       const int r1 = r0 + q.rows_per_item < q.m ? r0 + q.rows_per_item : q.m;
       switch (q.mode) {
-        case 0: run_rows<V, 1, false>(q, r0, r1, lane); break;
-        case 1: run_rows<V, 2, false>(q, r0, r1, lane); break;
-        case 2: run_rows<V, 2, true>(q, r0, r1, lane); break;
-        case 3: run_rows<V, 4, false>(q, r0, r1, lane); break;
-        default: run_rows<V, 4, true>(q, r0, r1, lane); break;
+        case 0: run_rows<V, 1, false, false>(q, r0, r1, lane); break;
+        case 1: run_rows<V, 2, false, false>(q, r0, r1, lane); break;
+        case 2: run_rows<V, 2, true, false>(q, r0, r1, lane); break;
+        case 3: run_rows<V, 4, false, false>(q, r0, r1, lane); break;
+        case 4: run_rows<V, 4, true, false>(q, r0, r1, lane); break;
+        case kBufferMode + 0: run_rows<V, 1, false, true>(q, r0, r1, lane); break;
+        case kBufferMode + 1: run_rows<V, 2, false, true>(q, r0, r1, lane); break;
+        case kBufferMode + 2: run_rows<V, 2, true, true>(q, r0, r1, lane); break;
+        case kBufferMode + 3: run_rows<V, 4, false, true>(q, r0, r1, lane); break;
+        default: run_rows<V, 4, true, true>(q, r0, r1, lane); break;
       }
     }
     }
@@ -290,8 +296,9 @@ struct Server {
         Matrix& A = *sm.A;
         const int rpi = rows_per_item(sm, p.k, kModeW[mode[i]]);
         const int32_t nitems = (int32_t)ceil_div(A.m, rpi);
+        const int32_t dmode = mode[i] | (rows_buffer_ok(A.nnz, A.n, p.ldx, vsize(dtype)) ? kBufferMode : 0);
         R[d++] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, p.X, p.Y, p.ldx, p.ldy, p.alpha, p.beta,
-                         (int32_t)A.m, p.k, lanes_of(sm, kModeW[mode[i]]), rpi, item, nitems, mode[i], {0, 0, 0}};
+                         (int32_t)A.m, p.k, lanes_of(sm, kModeW[mode[i]]), rpi, item, nitems, dmode, {0, 0, 0}};
         item += nitems;
       }
     }

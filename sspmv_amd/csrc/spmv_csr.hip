@@ -735,13 +735,13 @@ struct RowsArgs {
   int32_t m, k, lanes, rpi;
 };
 
-template <class V, int W, bool VEC>
+template <class V, int W, bool VEC, bool BUF>
 __global__ __launch_bounds__(kBlock) void csr_rows_kernel(RowsArgs a, int64_t nitems) {
   const int64_t item = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + threadIdx.x / 64;
   if (item >= nitems) return;  // whole waves leave together
   const int r0 = (int)(item * a.rpi);
   const int r1 = r0 + a.rpi < a.m ? r0 + a.rpi : a.m;
-  run_rows<V, W, VEC>(a, r0, r1, threadIdx.x & 63);
+  run_rows<V, W, VEC, BUF>(a, r0, r1, threadIdx.x & 63);
 }
 
 template <class V>
@@ -752,13 +752,16 @@ void rows_dispatch(Matrix& A, int lanes, int k, double alpha, const void* X, int
   const int64_t nitems = ceil_div(A.m, rpi);
   const RowsArgs a{A.rp.as<int>(), A.ci.as<int>(), A.val.p, X, Y, ldx, ldy, alpha, beta, (int32_t)A.m, k, lanes, rpi};
   const dim3 grid = grid1d(nitems, kBlock / 64);
+  const bool buf = rows_buffer_ok(A.nnz, A.n, ldx, (int64_t)sizeof(V));
+  void (*kern)(RowsArgs, int64_t);
   switch (mode) {
-    case 0: hipLaunchKernelGGL((csr_rows_kernel<V, 1, false>), grid, dim3(kBlock), 0, s, a, nitems); break;
-    case 1: hipLaunchKernelGGL((csr_rows_kernel<V, 2, false>), grid, dim3(kBlock), 0, s, a, nitems); break;
-    case 2: hipLaunchKernelGGL((csr_rows_kernel<V, 2, true>), grid, dim3(kBlock), 0, s, a, nitems); break;
-    case 3: hipLaunchKernelGGL((csr_rows_kernel<V, 4, false>), grid, dim3(kBlock), 0, s, a, nitems); break;
-    default: hipLaunchKernelGGL((csr_rows_kernel<V, 4, true>), grid, dim3(kBlock), 0, s, a, nitems); break;
+    case 0: kern = buf ? csr_rows_kernel<V, 1, false, true> : csr_rows_kernel<V, 1, false, false>; break;
+    case 1: kern = buf ? csr_rows_kernel<V, 2, false, true> : csr_rows_kernel<V, 2, false, false>; break;
+    case 2: kern = buf ? csr_rows_kernel<V, 2, true, true> : csr_rows_kernel<V, 2, true, false>; break;
+    case 3: kern = buf ? csr_rows_kernel<V, 4, false, true> : csr_rows_kernel<V, 4, false, false>; break;
+    default: kern = buf ? csr_rows_kernel<V, 4, true, true> : csr_rows_kernel<V, 4, true, false>; break;
   }
+  hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, s, a, nitems);
   LSP_LAUNCHED();
 }
 
