@@ -203,7 +203,7 @@ int lespmv_time_spmm(lespmv_matrix_t A, int fmt, int variant, int32_t k, double 
  * result of lespmv_spmv(LESPMV_CSR, LESPMV_VARIANT_MERGE) on the same matrix.
  * Calls no allocator: the tile partition and carries live in a fixed device
  * array of 3,670,272 bytes per GPU (held until the process ends), which limits
- * m + nnz to lespmv_csr_noalloc_max_items() = 134,217,728 merge-path items;
+ * m + nnz to lespmv_csr_noalloc_max_items() = 268,435,456 merge-path items;
  * larger inputs return LESPMV_ERR_UNSUPPORTED. stream NULL = the null stream.
  * Every call on one GPU shares that array: calls that overlap in time on one
  * GPU (other streams, other threads) race and must be serialised by the

@@ -203,10 +203,10 @@ void csr_analyse(Matrix& A, hipStream_t s);
 void csr_prepare(Matrix& A, int variant, hipStream_t s);
 void csr_release_aux(Matrix& A);
 // Merge-path CSR pieces on plain device pointers: the tile partition (tiles
-// of kMergeItems = 1024 merge-path items, coord int32[2 (tiles + 1)]) and the
+// of kMergeItems = 2048 merge-path items, coord int32[2 (tiles + 1)]) and the
 // tiles with their carry fix-up (carry_row int32[tiles], carry_val and
 // head_val V[tiles]).
-constexpr int kMergeItems = 1024;
+constexpr int kMergeItems = 2048;
 constexpr int kMergeThreads = 256;  // threads of a merge-path tile's workgroup
 void csr_merge_partition(int64_t tiles, int64_t m, int64_t nnz, const int* rp, int* coord, hipStream_t s);
 void csr_merge_tiles(int dtype, int64_t m, int64_t n, int64_t tiles, const int* rp, const int* ci, const void* val,

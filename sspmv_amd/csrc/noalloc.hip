@@ -22,7 +22,7 @@
 namespace lsp {
 namespace {
 
-constexpr int64_t kTiles = 131072;  // m + nnz <= 134,217,728 merge-path items
+constexpr int64_t kTiles = 131072;  // m + nnz <= 268,435,456 merge-path items
 constexpr size_t kCoordBytes = ((sizeof(int) * 2 * (kTiles + 1) + 255) / 256) * 256;
 constexpr size_t kCarryRowBytes = sizeof(int) * kTiles;
 constexpr size_t kScratchBytes = kCoordBytes + kCarryRowBytes + 2 * sizeof(double) * kTiles;  // 3,670,272

@@ -24,9 +24,10 @@ namespace {
 
 constexpr int kBlock = 256;
 // merge-path items (row ends + entries) per tile of the CSR kernel:
-// kMergeItems = 1024 on 256 threads (stencil 369 against 374 us with 2048,
-// FP32 lognormal 692 against 699; 2048 items on 512 threads: stencil 367
-// against 363, FP32 702 against 692, FP64 lognormal 841 against 842)
+// kMergeItems = 2048 on 256 threads, eight entries per thread (with x
+// gathered through a buffer resource; 27-point stencil 160^3 FP64 305 us
+// against 323 with 1024 on 256 and 313 with 2048 on 512, FP32 226 against
+// 260; banded rows FP32 206 against 262; lognormal rows 851 against 847)
 // items per tile of the column-panel kernel and threads of its workgroup:
 // FP64 2048 items on 512 threads (four entries per thread; 1024 items on 256
 // threads gave 555 against 566 us for 2048 on 256, whose LDS let fewer

@@ -69,7 +69,7 @@ def test_noalloc_two_calls_match_merge_entry(gpu, oracle, dt):
 def test_noalloc_limits_and_arguments(gpu):
     L = gpu.load()
     limit = L.lespmv_csr_noalloc_max_items()
-    assert limit == 131072 * 1024
+    assert limit == 131072 * 2048
     # beyond the fixed scratch: refused before any device access
     rc = L.lespmv_csr_spmv_noalloc(gpu.F64, limit, limit, 1, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), 1.0,
                                    C.c_void_p(16), 0.0, C.c_void_p(16), None)
