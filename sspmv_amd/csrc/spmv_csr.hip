@@ -29,13 +29,14 @@ constexpr int kBlock = 256;
 // against 323 with 1024 on 256 and 313 with 2048 on 512, FP32 226 against
 // 260; banded rows FP32 206 against 262; lognormal rows 851 against 847)
 // items per tile of the column-panel kernel and threads of its workgroup:
-// FP64 2048 items on 512 threads (four entries per thread; 1024 items on 256
-// threads gave 555 against 566 us for 2048 on 256, whose LDS let fewer
-// workgroups fit), FP32 2048 items on 256 threads (431 against 449 us with 1024)
+// 2048 items on 256 threads, eight entries per thread (FP64 lognormal rows
+// with x gathered through a buffer resource: 487 against 500 us on 512
+// threads; FP32 378 either way; 2048 is also the most a tile's row-end
+// words, one per lane of a wave, can cover)
 template <class V>
 constexpr int kPanelItems = 2048;
 template <class V>
-constexpr int kPanelThreads = sizeof(V) == 8 ? 512 : 256;
+constexpr int kPanelThreads = 256;
 
 template <int L, class V>
 __device__ __forceinline__ V group_sum(V v) {
