@@ -17,6 +17,7 @@
 //  * row (variant 1, any shape): one thread per row, walking its block row.
 #include <algorithm>
 
+#include "bsr_engine.hpp"
 #include "common.hpp"
 
 namespace lsp {
@@ -163,13 +164,8 @@ __global__ __launch_bounds__(kBlock) void bsr_spmm_kernel(int64_t m, int64_t n, 
   *yp = beta == V(0) ? alpha * sum : alpha * sum + beta * *yp;
 }
 
-// BSR 4x4 FP64 SpMM on the matrix cores: one wave per block row, one
-// v_mfma_f64_4x4x4f64 per block and group of 16 right-hand sides. The
-// instruction runs four independent 4x4x4 products; product q uses the lanes
-// 4q + x + 16y (x, y in 0..3) with A[i][k] at (x, y) = (i, k), B[k][j] at
-// (j, k) and D[i][j] at (j, i) (measured on gfx950). The four products share
-// the block (A) and take four consecutive groups of 4 right-hand sides
-// (B = the X rows of the block's 4 columns).
+// BSR 4x4 FP64 SpMM on the matrix cores (bsr_engine.hpp): one wave per block
+// row.
 __global__ __launch_bounds__(kBlock) void bsr44_spmm_mfma(int64_t m, int64_t n, int64_t mb, int k,
                                                           const int* __restrict__ rptr, const int* __restrict__ bcol,
                                                           const double* __restrict__ data,
@@ -177,34 +173,8 @@ __global__ __launch_bounds__(kBlock) void bsr44_spmm_mfma(int64_t m, int64_t n, 
                                                           double* __restrict__ Y, int64_t ldy, double alpha,
                                                           double beta) {
   const int64_t brow = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
-  if (brow >= mb) return;  // whole waves leave together: the MFMAs below need every lane
-  const int l = threadIdx.x & 63;
-  const int x = l & 3, q = (l >> 2) & 3, y = l >> 4;
-  const int j0 = 4 * q + x, j1 = j0 + 16;
-  double acc0 = 0.0, acc1 = 0.0;
-  const int e = rptr[brow + 1];
-  for (int b = rptr[brow]; b < e; b++) {
-    const double a = data[(int64_t)b * 16 + x * 4 + y];
-    const int64_t col = (int64_t)bcol[b] * 4 + y;
-    const bool cin = col < n;
-    const double b0 = (cin && j0 < k) ? X[col * ldx + j0] : 0.0;
-    acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b0, acc0, 0, 0, 0);
-    if (k > 16) {
-      const double b1 = (cin && j1 < k) ? X[col * ldx + j1] : 0.0;
-      acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b1, acc1, 0, 0, 0);
-    }
-  }
-  const int64_t row = brow * 4 + y;
-  if (row < m) {
-    if (j0 < k) {
-      double* p = Y + row * ldy + j0;
-      *p = beta == 0.0 ? alpha * acc0 : alpha * acc0 + beta * *p;
-    }
-    if (j1 < k) {
-      double* p = Y + row * ldy + j1;
-      *p = beta == 0.0 ? alpha * acc1 : alpha * acc1 + beta * *p;
-    }
-  }
+  if (brow >= mb) return;  // whole waves leave together: the MFMAs need every lane
+  bsr44_mfma_row(rptr, bcol, data, m, n, k, X, ldx, Y, ldy, alpha, beta, brow, threadIdx.x & 63);
 }
 
 int bits_for(uint64_t v) {

@@ -1,6 +1,7 @@
 """BSR 4x4 SpMM on the matrix cores (v_mfma_f64_4x4x4f64) against the scalar
 BSR kernel and CSR SpMM, including ragged edges (rows and columns not a
-multiple of 4) and k not a multiple of 4."""
+multiple of 4), k not a multiple of 4, and k beyond 32 (up to four groups of
+16 right-hand sides per block)."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,7 @@ from tests.matrices import random_csr
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("k", [2, 3, 4, 5, 16, 17, 31, 32])
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 16, 17, 31, 32, 33, 48, 63, 64])
 def test_bsr44_mfma_spmm(gpu, k):
     lib = gpu
     cases = [lib.generate(lib.GEN_BLOCKRAND, [300, 280, 4, 4, 8], 7),
