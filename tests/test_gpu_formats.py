@@ -167,7 +167,7 @@ def test_not_converted_is_an_error(gpu):
 
 
 @pytest.mark.parametrize("fmt", [ELL, SELL, SELL_C_SIGMA, SELL_C_R, COO, DIA, BSR, CSR5])
-@pytest.mark.parametrize("k", [1, 4, 8, 16, 32])
+@pytest.mark.parametrize("k", [1, 4, 8, 16, 32, 33, 64])
 def test_format_spmm_columns_equal_spmv(gpu, oracle, fmt, k):
     if not available(gpu, fmt):
         pytest.skip(f"format {fmt} not built in this library")
