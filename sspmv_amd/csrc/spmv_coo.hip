@@ -88,9 +88,10 @@ __global__ __launch_bounds__(kBlock) void coo_kernel(int64_t nnz, const int* __r
   for (int r = 0; r < kRounds; r++) {
     const int64_t i = w0 + (int64_t)r * 64 + lane;
     const bool valid = i < w1;
-    keys[r] = valid ? row[i] : -2 - lane;  // distinct keys for lanes past the end
-    cols[r] = valid ? col[i] : 0;
-    prods[r] = valid ? val[i] : V(0);
+    // read once: non-temporal loads keep x in the caches
+    keys[r] = valid ? __builtin_nontemporal_load(row + i) : -2 - lane;  // distinct keys for lanes past the end
+    cols[r] = valid ? __builtin_nontemporal_load(col + i) : 0;
+    prods[r] = valid ? __builtin_nontemporal_load(val + i) : V(0);
   }
 #pragma unroll
   for (int r = 0; r < kRounds; r++) {

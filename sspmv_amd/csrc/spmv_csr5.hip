@@ -233,14 +233,15 @@ __global__ __launch_bounds__(kBlock) void csr5_tiles(int64_t p, int sigma_rt, in
   const bool first_direct = (__shfl(d, 0, W) >> (31 - bit_all)) & 1;
   const int* tc = col + t * tile;
   const V* tv = val + t * tile;
-  // all loads of the lane issued before use
+  // all loads of the lane issued before use; the tile's columns and values
+  // are read once (non-temporal loads keep x in the caches)
   int c[NS];
   V v[NS];
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     if (i < sigma) {
-      c[i] = tc[(int64_t)i * W + l];
-      v[i] = tv[(int64_t)i * W + l];
+      c[i] = __builtin_nontemporal_load(tc + (int64_t)i * W + l);
+      v[i] = __builtin_nontemporal_load(tv + (int64_t)i * W + l);
     }
   }
 #pragma unroll
