@@ -161,7 +161,7 @@ struct Cluster {
   int base;        // LDS element base of this cluster's window
 };
 
-template <class V>
+template <class V, bool NT>
 __global__ __launch_bounds__(kBlock) void dia_window(int64_t m, int64_t n, int nd, int64_t stride,
                                                      const int64_t* __restrict__ offs,
                                                      const int* __restrict__ lds_off, const V* __restrict__ data,
@@ -185,7 +185,10 @@ __global__ __launch_bounds__(kBlock) void dia_window(int64_t m, int64_t n, int n
   V sum = 0;
   for (int d = 0; d < nd; d++) {
     const int64_t j = i + offs[d];
-    if (j >= 0 && j < n) sum += data[(int64_t)d * stride + i] * win[lds_off[d] + threadIdx.x];
+    if (j >= 0 && j < n) {
+      const V* dp = data + (int64_t)d * stride + i;
+      sum += (NT ? __builtin_nontemporal_load(dp) : *dp) * win[lds_off[d] + threadIdx.x];
+    }
   }
   y[i] = beta == V(0) ? alpha * sum : alpha * sum + beta * y[i];
 }
@@ -266,9 +269,16 @@ void plan_windows(DiaDev& d, const std::vector<int64_t>& offs, size_t vbytes, hi
 
 template <class V>
 void run(Matrix& A, DiaDev& d, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  // diagonals beyond the 256 MB last-level cache are streamed with
+  // non-temporal loads, so they do not push x out of the caches (27-point
+  // stencil 160^3: FP64 192 -> 179 us, FP32 94 -> 89 us); smaller ones keep
+  // normal loads, which let repeated calls find part of them in that cache
+  // (2-D Laplacian 1024^2 over four rotated copies: 11.7 us against 13.7)
+  const bool nt = (double)d.ndiag * (double)d.stride * (double)sizeof(V) > 256.0 * (1 << 20);
   if (variant == LESPMV_VARIANT_MERGE || (variant == LESPMV_VARIANT_AUTO && d.nclusters > 0)) {
     if (d.nclusters > 0) {
-      hipLaunchKernelGGL((dia_window<V>), grid1d(A.m, kBlock), dim3(kBlock), d.lds_elems * sizeof(V), s, A.m, A.n,
+      hipLaunchKernelGGL((nt ? dia_window<V, true> : dia_window<V, false>), grid1d(A.m, kBlock), dim3(kBlock),
+                         d.lds_elems * sizeof(V), s, A.m, A.n,
                          d.ndiag, d.stride, d.offsets.as<int64_t>(), d.lds_off.as<int>(), d.data.as<V>(),
                          static_cast<const V*>(x), static_cast<V*>(y), (V)alpha, (V)beta, d.nclusters,
                          d.clo.as<int64_t>(), d.cwidth.as<int>(), d.cbase.as<int>());
@@ -277,12 +287,6 @@ void run(Matrix& A, DiaDev& d, int variant, double alpha, const void* x, double 
     }
   }
   constexpr int R = kDiaRows<V>;
-  // diagonals beyond the 256 MB last-level cache are streamed with
-  // non-temporal loads, so they do not push x out of the caches (27-point
-  // stencil 160^3: FP64 192 -> 179 us, FP32 94 -> 89 us); smaller ones keep
-  // normal loads, which let repeated calls find part of them in that cache
-  // (2-D Laplacian 1024^2 over four rotated copies: 11.7 us against 13.7)
-  const bool nt = (double)d.ndiag * (double)d.stride * (double)sizeof(V) > 256.0 * (1 << 20);
   hipLaunchKernelGGL((nt ? dia_vector<V, R, kDiaUnroll<V>, true> : dia_vector<V, R, kDiaUnroll<V>, false>),
                      grid1d(ceil_div(A.m, R), kBlock), dim3(kBlock), 0, s, A.m, A.n, d.ndiag, d.stride,
                      d.offsets.as<int64_t>(), d.data.as<V>(), static_cast<const V*>(x), static_cast<V*>(y), (V)alpha,

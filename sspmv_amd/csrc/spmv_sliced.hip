@@ -77,14 +77,19 @@ __global__ void sliced_fill(int64_t m, int C, const int* __restrict__ rp, const 
 // that sliced_fixup adds in unit order. One thread per (unit, row of chunk);
 // padding sits at the end of each row, so a col < 0 ends the row.
 constexpr int kUnitSlots = 64;
-// slots per pipelined step: 12 (27-point stencil 280 us against 286 with 8
-// and 282 with 16; lognormal rows 912 against 928 with 8). Column and value
-// slots are read once, with non-temporal loads, so they do not push x out of
-// the caches (27-point stencil 160^3 FP64 278 -> 263 us, FP32 170 -> 154 us;
-// lognormal rows 905 -> 887 us; 2-D Laplacian 1024^2 24.1 -> 21.6 us)
-constexpr int kUnroll = 12;
+// Column and value slots are read once, with non-temporal loads, so they do
+// not push x out of the caches (27-point stencil 160^3 FP64 278 -> 263 us,
+// FP32 170 -> 154 us; lognormal rows 905 -> 887 us; 2-D Laplacian 1024^2
+// 24.1 -> 21.6 us). Slots per pipelined step (kUnroll) follow the layout's
+// slot-weighted mean unit width w (sum of w^2 over sum of w, computed by
+// plan_units): 6 up to 8 positions (2-D Laplacian, 5: 16.9 us against 21.6
+// with 12; the registers of a short step leave room for 8 waves per SIMD),
+// else 12 (27-point stencil, 27: 263 us against 265 with 6 and 267 with 16;
+// on lognormal rows, about 38, 24 gained 1-2 % for SELL-C-sigma and lost 6 %
+// for S-ELL, whose unsorted chunks leave most lanes idle in wide steps).
+inline int unroll_for(double w) { return w <= 8.0 ? 6 : 12; }
 
-template <class V>
+template <class V, int kUnroll>
 __global__ __launch_bounds__(kBlock) void sliced_spmv_kernel(int64_t m, int64_t nunits, int C,
                                                              const int4* __restrict__ units,
                                                              const int64_t* __restrict__ coff,
@@ -193,6 +198,7 @@ __global__ __launch_bounds__(kBlock) void sliced_spmm_kernel(int64_t m, int k, i
 
 struct SlicedUnits : SlicedData {
   int64_t nunits = 0, nsplit = 0;
+  int unroll = 12;  // slots per pipelined step of the SpMV kernel
   DBuf units;    // int4[nunits]: chunk, j0, j1, partial slot or -1
   DBuf split;    // int4[nsplit]: chunk, first partial slot, unit count
   DBuf partial;  // V[partial slots * C]
@@ -219,6 +225,12 @@ void plan_units(SlicedUnits& d, const std::vector<int>& clen, size_t vbytes, hip
                                                             pslots + q));
     pslots += cnt;
   }
+  double sw = 0, sw2 = 0;
+  for (const int4& u : units) {
+    sw += (double)(u.z - u.y);
+    sw2 += (double)(u.z - u.y) * (double)(u.z - u.y);
+  }
+  d.unroll = unroll_for(sw > 0 ? sw2 / sw : 0.0);
   d.nunits = (int64_t)units.size();
   d.nsplit = (int64_t)split.size();
   d.units.alloc(sizeof(int4) * std::max<size_t>(units.size(), 1));
@@ -360,7 +372,8 @@ void convert_sliced(Matrix& A, int fmt, const lespmv_params_t& p, hipStream_t s)
 
 template <class V>
 void sliced_run(Matrix& A, SlicedUnits& d, double alpha, const void* x, double beta, void* y, hipStream_t s) {
-  hipLaunchKernelGGL((sliced_spmv_kernel<V>), grid1d(d.nunits * d.C, kBlock), dim3(kBlock), 0, s, A.m, d.nunits, d.C,
+  auto kern = d.unroll == 6 ? sliced_spmv_kernel<V, 6> : sliced_spmv_kernel<V, 12>;
+  hipLaunchKernelGGL(kern, grid1d(d.nunits * d.C, kBlock), dim3(kBlock), 0, s, A.m, d.nunits, d.C,
                      d.units.as<int4>(), d.chunk_off.as<int64_t>(), d.reorder.as<int>(), d.col.as<int>(),
                      d.val.as<V>(), static_cast<const V*>(x), static_cast<V*>(y), d.partial.as<V>(), (V)alpha,
                      (V)beta, (int)((int64_t)d.sigma * kXcds * 8 <= A.m));  // XCD-contiguous rows unless sorted globally
