@@ -277,17 +277,14 @@ __device__ __forceinline__ void merge_tile(int b, int64_t m, const int* __restri
   }
 }
 
-template <class V>
+template <class V, bool NT>
 __global__ __launch_bounds__(kMergeThreads) void csr_merge_kernel(int64_t m, int64_t n, const int* __restrict__ rp,
                                                            const int* __restrict__ ci, const V* __restrict__ val,
                                                            const V* __restrict__ x, V* __restrict__ y, V alpha,
                                                            V beta, const int* __restrict__ coord,
                                                            int* __restrict__ carry_row, V* __restrict__ carry_val,
                                                            V* __restrict__ head_val) {
-  // the tile's indices, values and row pointers are read once: non-temporal
-  // loads (27-point stencil 160^3 FP64 284 -> 278 us, 2-D Laplacian 1024^2
-  // 23.0 -> 22.2 us)
-  merge_tile<V, true, kMergeItems, false, false, kMergeThreads>(
+  merge_tile<V, NT, kMergeItems, false, false, kMergeThreads>(
       (int)xcd_block(blockIdx.x, gridDim.x), m, rp, ci, val, x, y, alpha, beta, coord, carry_row, carry_val, head_val,
       nullptr, nullptr, nullptr, 0, n * (int64_t)sizeof(V));
 }
@@ -579,7 +576,15 @@ void merge_launch(int64_t m, int64_t n, int64_t tiles, const int* rp, const int*
                   double alpha, double beta, const int* coord, int* carry_row, V* carry_val, V* head_val,
                   hipStream_t s) {
   if (tiles == 0) return;
-  hipLaunchKernelGGL((csr_merge_kernel<V>), dim3((unsigned)tiles), dim3(kMergeThreads), 0, s, m, n, rp, ci, val,
+  // a matrix beyond the 256 MB last-level cache is streamed with
+  // non-temporal loads, so its arrays do not push x out of the caches
+  // (27-point stencil 160^3 FP64 284 -> 278 us); one that fits keeps normal
+  // loads, which let repeated calls find it in that cache (matrices of 10k to
+  // 2M rows: 5 % slower with non-temporal loads)
+  const int64_t nnz = tiles * (int64_t)kMergeItems - m;  // upper bound (the last tile may be short)
+  const bool nt = (double)nnz * (double)(sizeof(V) + 4) + 4.0 * (double)m > 256.0 * (1 << 20);
+  hipLaunchKernelGGL((nt ? csr_merge_kernel<V, true> : csr_merge_kernel<V, false>), dim3((unsigned)tiles),
+                     dim3(kMergeThreads), 0, s, m, n, rp, ci, val,
                      static_cast<const V*>(x), static_cast<V*>(y), (V)alpha, (V)beta, coord, carry_row, carry_val,
                      head_val);
   LSP_LAUNCHED();

@@ -377,7 +377,7 @@ def retime(db_path, out, columns, iters=10, max_nnz=6e6, n_max=2e6, rows=None, l
             db["times"][r, c] = cands[j].ms if cands[j].status == 0 else np.inf
             db["convert_ms"][r, c] = cands[j].convert_ms
         del M, A
-        if (r + 1) % 500 == 0 or r + 1 == len(seeds):
+        if (r + 1) % 100 == 0 or r + 1 == len(seeds):
             log(f"retime {r + 1}/{len(seeds)} {time.time() - t0:.0f}s", flush=True)
     np.savez_compressed(out, **db)
     return db
