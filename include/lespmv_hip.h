@@ -342,8 +342,11 @@ typedef struct {
 int lespmv_server_create(lespmv_server_t* out, int32_t dtype, int64_t hbm_budget, int32_t max_batch, int32_t nslots);
 int lespmv_server_destroy(lespmv_server_t S);
 /* Copies a host CSR in; lanes = lanes per row (rows per wave = 64 / lanes; power of two <= 64) for every
- * request, or 0 = autotune: SpMV lanes from the CSR vector candidates, SpMM lanes from timing the row
- * engine at k = 2 and k = 16 (requests with k = 2..3 and k >= 4 use those). */
+ * request, all through the CSR row engine, or 0 = autotune: SpMV lanes from the CSR vector candidates,
+ * SpMM lanes from timing the row engine at k = 2 and k = 16 (requests with k = 2..3 and k >= 4 use
+ * those); then, timed per call against those, a sliced layout for k = 1 (S-ELL or SELL-C-4096, C in
+ * {32, 64}, for rows of at most 64 nonzeros) and 4x4 BSR on the matrix cores (FP64, at most 1.5 stored
+ * values per nonzero) for k from 5, 9 or 17 up, each kept where it is at least 3 % faster. */
 int lespmv_server_add_csr(lespmv_server_t S, int64_t m, int64_t n, int64_t nnz, const int32_t* row_offset,
                           const int32_t* col_index, const void* values, int32_t lanes, int32_t* id);
 int lespmv_server_remove(lespmv_server_t S, int32_t id);
@@ -352,6 +355,19 @@ int lespmv_server_matrix(lespmv_server_t S, int32_t id, lespmv_matrix_t* A, int3
 /* Lanes per row the server uses for this matrix's requests with k right-hand sides (the same product
  * through lespmv_spmm with CSR variant 16 + log2(lanes) gives bitwise the server's result). */
 int lespmv_server_lanes(lespmv_server_t S, int32_t id, int32_t k, int32_t* lanes);
+/* Sets the layouts a resident matrix is served from (no request of it may be in flight):
+ * fmt = LESPMV_SELL or LESPMV_SELL_C_SIGMA: k = 1 requests through that sliced layout (params: chunk
+ *   32 or 64, sigma, alignment; LESPMV_ERR_UNSUPPORTED when a chunk holds more than 64 slots);
+ * fmt = LESPMV_BSR: requests with k >= k_from (5..32) through 4x4 BSR on the matrix cores (FP64 servers;
+ *   params->max_fill bounds the fill when params is given);
+ * fmt = LESPMV_CSR: every request through the row engine again (the other layouts released).
+ * A layout's device bytes count against the budget (LESPMV_ERR_OOM when no idle matrix can make room).
+ * The layouts live on the matrix handle (lespmv_server_matrix): lespmv_spmv of the sliced format and
+ * lespmv_spmm of BSR (k > 4) on it give bitwise the server's results. */
+int lespmv_server_use(lespmv_server_t S, int32_t id, int32_t fmt, const lespmv_params_t* params, int32_t k_from);
+/* The format a request of this matrix with k right-hand sides runs on: LESPMV_CSR (row engine),
+ * LESPMV_SELL / LESPMV_SELL_C_SIGMA or LESPMV_BSR. */
+int lespmv_server_format(lespmv_server_t S, int32_t id, int32_t k, int32_t* fmt);
 int lespmv_server_submit(lespmv_server_t S, int32_t id, int32_t k, double alpha, const void* X, int64_t ldx,
                          double beta, void* Y, int64_t ldy, int64_t* request_id);
 /* Completed requests since the last poll (wait = 1 blocks until at least one

@@ -88,6 +88,14 @@ int main(int argc, char** argv) {
   const double budget_gb = std::stod(arg(argc, argv, "--budget-gb", "200"));
   const int nslots = std::stoi(arg(argc, argv, "--slots", "4"));
   const int fixed_lanes = std::stoi(arg(argc, argv, "--lanes", "0"));  // 0: the server tunes lanes per matrix
+  // auto: the server's tuned layouts (sliced for k = 1, BSR 4x4 from a k up) where they timed faster;
+  // csr: every request on the CSR row engine (lanes still tuned); bsr: BSR 4x4 for k >= 9 wherever its
+  // fill stays within 2 stored values per nonzero, else as auto
+  const std::string formats = arg(argc, argv, "--formats", "auto");
+  if (formats != "auto" && formats != "csr" && formats != "bsr") {
+    std::fprintf(stderr, "--formats is auto, csr or bsr\n");
+    return 2;
+  }
   std::vector<int> ks;  // right-hand-side counts to draw from (default 1..32)
   {
     const std::string r = arg(argc, argv, "--ks", "");
@@ -131,6 +139,7 @@ int main(int argc, char** argv) {
   // ---- resident matrices and per-call handles ----
   std::vector<Mat> mats(nmat);
   int64_t total_nnz = 0, total_rows = 0;
+  int sliced_mats = 0, bsr_mats = 0;  // matrices served from a sliced layout at k = 1, from BSR at k = 32
   const double t_build = now_us();
   for (int i = 0; i < nmat; i++) {
     const uint64_t h = mix(seed * 7919 + (uint64_t)i);
@@ -162,7 +171,18 @@ int main(int argc, char** argv) {
     M.nnz = c.nnz;
     M.fam = fam;
     CHECK(lespmv_server_add_csr(S, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, fixed_lanes, &M.id));
+    if (formats == "csr") CHECK(lespmv_server_use(S, M.id, LESPMV_CSR, nullptr, 0));
+    if (formats == "bsr") {
+      lespmv_params_t bp{};
+      bp.max_fill = 2;
+      (void)lespmv_server_use(S, M.id, LESPMV_BSR, &bp, 9);  // refused beyond the fill limit
+    }
     CHECK(lespmv_server_matrix(S, M.id, nullptr, &M.lanes));
+    int32_t f1 = 0, f32 = 0;
+    CHECK(lespmv_server_format(S, M.id, 1, &f1));
+    CHECK(lespmv_server_format(S, M.id, 32, &f32));
+    sliced_mats += f1 != LESPMV_CSR;
+    bsr_mats += f32 == LESPMV_BSR;
     CHECK(lespmv_create_csr(&M.h, LESPMV_F64, c.m, c.n, c.nnz, c.row_offset, c.col_index, c.values, 0));
     lespmv_host_csr_free(&c);
     CHECK(lespmv_malloc(&M.X, 8 * M.n * 32));
@@ -333,10 +353,10 @@ int main(int argc, char** argv) {
   CHECK(lespmv_server_stats(S, &st));
 
   std::printf("{\"matrices\": %d, \"requests\": %d, \"bytes_per_request\": %.0f, \"total_rows\": %lld, \"total_nnz\": %lld, "
-              "\"resident_bytes\": %lld, \"build_s\": %.1f, \"batches\": %lld, \"server_saturation\": %s, "
-              "\"server_poisson\": [",
+              "\"resident_bytes\": %lld, \"build_s\": %.1f, \"formats\": \"%s\", \"sliced_k1_matrices\": %d, "
+              "\"bsr_k32_matrices\": %d, \"batches\": %lld, \"server_saturation\": %s, \"server_poisson\": [",
               nmat, nreq, bytes / nreq, (long long)total_rows, (long long)total_nnz, (long long)st.bytes_used, build_s,
-              (long long)st.batches, sat.c_str());
+              formats.c_str(), sliced_mats, bsr_mats, (long long)st.batches, sat.c_str());
   for (size_t i = 0; i < loads.size(); i++) std::printf("%s%s", i ? ", " : "", loads[i].c_str());
   std::printf("], \"unloaded\": %s, \"percall_sync\": %s, \"percall_async\": %s, \"percall_engine_sync\": %s, "
               "\"percall_engine_async\": %s}\n",

@@ -4,22 +4,30 @@
 // matrix per process, test/benchmark_spmv_<fmt>.cpp).
 //
 // Resident matrices are the server's own CSR copies with a lanes-per-row value
-// from the autotuner. A batch is a table of request descriptors (matrix arrays,
-// X, Y, k, alpha, beta, rows per work item, first work item); a request's work
-// items are consecutive row ranges, sized on the host from the matrix's mean
-// row length and k so that an item is a few passes of ~kItemIters loads per
-// lane. Each request is placed on one XCD (largest work first, least loaded
-// XCD): its items form that XCD's queue, so its X is gathered into one L2.
+// from the autotuner, plus, where the autotuner timed them faster, a sliced
+// layout (S-ELL or SELL-C-sigma, C in {32, 64}, every chunk at most
+// kUnitSlots wide) for k = 1 and a 4x4 BSR layout (FP64, matrix cores) for k
+// from a timed threshold up. A batch is a table of request descriptors (matrix
+// arrays, X, Y, k, alpha, beta, units per work item, first work item); a
+// request's work items are consecutive ranges of rows (CSR), chunks (sliced) or
+// block rows (BSR), sized on the host from the matrix's mean row length and k
+// so that an item is a few passes of ~kItemIters loads per lane. Each request
+// is placed on one XCD (largest work first, least loaded XCD): its items form
+// that XCD's queue, so its X is gathered into one L2.
 // The persistent kernel's waves take a few items at a time (the batch's grab
 // size, about eight grabs per wave) from their own XCD's counter and then help
 // the other queues, find the request by a search over the first item numbers
-// held in LDS, and compute the rows for all k right-hand sides: a lane covers
+// held in LDS, and compute the item. CSR rows run on the row engine for all k
+// right-hand sides: a lane covers
 // W right-hand sides (1, 2 or 4, one vector access when aligned), a row gets G
 // lanes (G = max(lanes, kk), kk the lanes that cover k), lane t handles
 // right-hand sides (t mod kk)*W.. and every (G/kk)-th nonzero, and the partial
-// sums meet in xor shuffles. W, G and the order of every sum depend only on
-// the request, never on the batch or on which wave runs an item, so a
-// request's result is bitwise the same alone and in any mix.
+// sums meet in xor shuffles. Sliced chunks take one thread per row
+// (sliced_engine.hpp, 64 / C chunks per wave), BSR block rows one wave each
+// (bsr_engine.hpp). Formats, lane layouts and the order of every sum depend
+// only on the request, never on the batch or on which wave runs an item, so a
+// request's result is bitwise the same alone and in any mix, and the same as
+// the per-call kernel on the same layout.
 //
 // Each in-flight slot owns a stream, pinned host table, device table (per-XCD
 // counters, header, descriptors) and captured graphs: copy the table in
@@ -30,8 +38,10 @@
 #include <deque>
 #include <vector>
 
+#include "bsr_engine.hpp"
 #include "common.hpp"
 #include "rows_engine.hpp"
+#include "sliced_engine.hpp"
 
 namespace lsp {
 
@@ -41,18 +51,65 @@ constexpr int kMaxBatch = 256;
 constexpr int kServeBlock = 256;
 
 struct alignas(16) ReqDesc {
-  const int* rp;
-  const int* ci;
-  const void* val;
+  const int* rp;        // CSR row offsets | sliced chunk widths | BSR block-row offsets
+  const int* ci;        // CSR columns | sliced slot columns | BSR block columns
+  const void* val;      // CSR values | sliced slot values | BSR block values (16 per block, row-major)
   const void* X;
   void* Y;
+  const int64_t* coff;  // sliced: chunk offsets
+  const int* order;     // sliced: original row of each sorted row (null: natural order)
   int64_t ldx, ldy;
   double alpha, beta;
-  int32_t m, k, lanes, rows_per_item;
+  int32_t m, n, k, lanes;
+  int32_t rows_per_item, units;  // units per work item and in all: rows, chunks or block rows
   int32_t item_begin, nitems;
-  int32_t mode, pad[3];  // right-hand sides per lane and vector access (rhs_mode), + kBufferMode for buffer loads
+  int32_t mode, C, pad[2];  // mode: rows_mode (+ kBufferMode for buffer loads), kSellMode or kBsrMode; C: chunk
 };
 constexpr int32_t kBufferMode = 8;
+constexpr int32_t kSellMode = 16;  // k = 1 through a sliced layout
+constexpr int32_t kBsrMode = 17;   // 4x4 BSR on the matrix cores (FP64)
+constexpr int kSellSteps = 6;      // slots per pipelined step (keeps the packed kernel's registers)
+
+// Generic pointers read from a descriptor, told to the compiler as global
+// ones, so it issues global (not flat) loads.
+template <class T>
+__device__ __forceinline__ T* global_ptr(T* p) {
+  return (T*)(LSP_GLOBAL_AS T*)p;
+}
+
+// Chunks [c0, c1) of a sliced request (k = 1): lane t takes row t mod C of
+// chunk c0 + t / C, then the chunk 64 / C further on.
+template <class V>
+__device__ __forceinline__ void run_sell(const ReqDesc& q, int c0, int c1, int lane) {
+  const int C = q.C, per = 64 / C, r = lane % C;
+  const int* __restrict__ width = global_ptr(q.rp);
+  const int* __restrict__ col = global_ptr(q.ci);
+  const V* __restrict__ val = global_ptr((const V*)q.val);
+  const int64_t* __restrict__ coff = global_ptr(q.coff);
+  const int* __restrict__ order = q.order ? global_ptr(q.order) : nullptr;
+  const V* __restrict__ x = global_ptr((const V*)q.X);
+  V* __restrict__ y = global_ptr((V*)q.Y);
+  const V alpha = (V)q.alpha, beta = (V)q.beta;
+  for (int c = c0 + lane / C; c < c1; c += per) {
+    const int64_t row = (int64_t)c * C + r;
+    if (row >= q.m) continue;
+    const int64_t off = coff[c];
+    const V sum = sliced_row_sum<V, kSellSteps>(col + off + r, val + off + r, C, 0, width[c], x, q.ldx);
+    V* out = y + (order ? (int64_t)order[row] : row) * q.ldy;
+    *out = beta == V(0) ? alpha * sum : alpha * sum + beta * *out;
+  }
+}
+
+// Block rows [b0, b1) of a 4x4 BSR request, one after another on the whole wave.
+__device__ __forceinline__ void run_bsr(const ReqDesc& q, int b0, int b1, int lane) {
+  const int* rptr = global_ptr(q.rp);
+  const int* bcol = global_ptr(q.ci);
+  const double* data = global_ptr((const double*)q.val);
+  const double* X = global_ptr((const double*)q.X);
+  double* Y = global_ptr((double*)q.Y);
+  for (int b = b0; b < b1; b++)
+    bsr44_mfma_row(rptr, bcol, data, q.m, q.n, q.k, X, q.ldx, Y, q.ldy, q.alpha, q.beta, b, lane);
+}
 
 // Each request is placed on one XCD: its work items form that XCD's queue
 // [xbegin[x], xbegin[x + 1]) with its own counter, so the request's X is
@@ -114,7 +171,7 @@ __global__ __launch_bounds__(kServeBlock) void packed_kernel(const unsigned char
       lo = __builtin_amdgcn_readfirstlane(lo);
       const ReqDesc& q = R[lo];
       const int r0 = (item - s_begin[lo]) * q.rows_per_item;
-      const int r1 = r0 + q.rows_per_item < q.m ? r0 + q.rows_per_item : q.m;
+      const int r1 = r0 + q.rows_per_item < q.units ? r0 + q.rows_per_item : q.units;
       switch (q.mode) {
         case 0: run_rows<V, 1, false, false>(q, r0, r1, lane); break;
         case 1: run_rows<V, 2, false, false>(q, r0, r1, lane); break;
@@ -125,7 +182,11 @@ __global__ __launch_bounds__(kServeBlock) void packed_kernel(const unsigned char
         case kBufferMode + 1: run_rows<V, 2, false, true>(q, r0, r1, lane); break;
         case kBufferMode + 2: run_rows<V, 2, true, true>(q, r0, r1, lane); break;
         case kBufferMode + 3: run_rows<V, 4, false, true>(q, r0, r1, lane); break;
-        default: run_rows<V, 4, true, true>(q, r0, r1, lane); break;
+        case kBufferMode + 4: run_rows<V, 4, true, true>(q, r0, r1, lane); break;
+        case kSellMode: run_sell<V>(q, r0, r1, lane); break;
+        default:
+          if constexpr (sizeof(V) == 8) run_bsr(q, r0, r1, lane);  // kBsrMode: FP64 servers only
+          break;
       }
     }
     }
@@ -144,9 +205,18 @@ struct ServerMat {
   double avg_row = 0.0;
   int32_t lanes = 1;                   // k = 1
   int32_t lanes_w[3] = {1, 1, 1};      // per right-hand sides per lane W = 1, 2, 4
+  int32_t sell_fmt = -1;               // k = 1 through this sliced layout (S-ELL or SELL-C-sigma); -1: row engine
+  int32_t bsr_from_k = 0;              // k >= this through the 4x4 BSR layout on the matrix cores; 0: never
   int64_t bytes = 0, last_use = 0, pending = 0;
   bool live = false;
 };
+
+// The kernel a request runs on: 0 the CSR row engine, 1 the sliced layout, 2 BSR 4x4.
+int request_kind(const ServerMat& sm, int k) {
+  if (k == 1 && sm.sell_fmt >= 0) return 1;
+  if (sm.bsr_from_k > 0 && k >= sm.bsr_from_k) return 2;
+  return 0;
+}
 
 struct Pending {
   int64_t id;
@@ -258,8 +328,9 @@ struct Server {
     int64_t total = 0;
     while (n < (int)std::min<size_t>(queue.size(), (size_t)max_batch)) {
       const Pending& p = queue[n];
-      const int64_t m = mats[p.mat].A->m;
-      const int64_t add = ceil_div(m, rows_per_item(mats[p.mat], p.k, kModeW[rhs_mode(p)]));
+      int64_t units = 0;
+      const int per = item_units(p, &units);
+      const int64_t add = ceil_div(units, (int64_t)per);
       if (n > 0 && total + add >= INT32_MAX) break;
       total += add;
       n++;
@@ -275,7 +346,7 @@ struct Server {
       const Pending& p = batch[i];
       const Matrix& A = *mats[p.mat].A;
       mode[i] = rhs_mode(p);
-      work[i] = (double)A.m + (double)A.nnz * (double)ceil_div(p.k, kModeW[mode[i]]);
+      work[i] = (double)A.m + (double)A.nnz * (double)ceil_div(p.k, kModeW[mode[i]]);  // an estimate for every kind
       order[i] = i;
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
@@ -294,11 +365,47 @@ struct Server {
         const Pending& p = batch[i];
         ServerMat& sm = mats[p.mat];
         Matrix& A = *sm.A;
-        const int rpi = rows_per_item(sm, p.k, kModeW[mode[i]]);
-        const int32_t nitems = (int32_t)ceil_div(A.m, rpi);
-        const int32_t dmode = mode[i] | (rows_buffer_ok(A.nnz, A.n, p.ldx, vsize(dtype)) ? kBufferMode : 0);
-        R[d++] = ReqDesc{A.rp.as<int>(), A.ci.as<int>(), A.val.p, p.X, p.Y, p.ldx, p.ldy, p.alpha, p.beta,
-                         (int32_t)A.m, p.k, lanes_of(sm, kModeW[mode[i]]), rpi, item, nitems, dmode, {0, 0, 0}};
+        int64_t units = 0;
+        const int per = item_units(p, &units);
+        const int32_t nitems = (int32_t)ceil_div(units, per);
+        ReqDesc q{};
+        q.X = p.X;
+        q.Y = p.Y;
+        q.ldx = p.ldx;
+        q.ldy = p.ldy;
+        q.alpha = p.alpha;
+        q.beta = p.beta;
+        q.m = (int32_t)A.m;
+        q.n = (int32_t)A.n;
+        q.k = p.k;
+        q.lanes = lanes_of(sm, kModeW[mode[i]]);
+        q.rows_per_item = per;
+        q.units = (int32_t)units;
+        q.item_begin = item;
+        q.nitems = nitems;
+        const int kind = request_kind(sm, p.k);
+        if (kind == 1) {
+          const auto& L = *static_cast<const SlicedData*>(A.fmt[sm.sell_fmt].get());
+          q.rp = L.chunk_len.as<int>();
+          q.ci = L.col.as<int>();
+          q.val = L.val.p;
+          q.coff = L.chunk_off.as<int64_t>();
+          q.order = L.reorder.p ? L.reorder.as<int>() : nullptr;
+          q.mode = kSellMode;
+          q.C = L.C;
+        } else if (kind == 2) {
+          const auto& B = *static_cast<const BsrData*>(A.fmt[LESPMV_BSR].get());
+          q.rp = B.row_ptr.as<int>();
+          q.ci = B.bcol.as<int>();
+          q.val = B.data.p;
+          q.mode = kBsrMode;
+        } else {
+          q.rp = A.rp.as<int>();
+          q.ci = A.ci.as<int>();
+          q.val = A.val.p;
+          q.mode = mode[i] | (rows_buffer_ok(A.nnz, A.n, p.ldx, vsize(dtype)) ? kBufferMode : 0);
+        }
+        R[d++] = q;
         item += nitems;
       }
     }
@@ -331,6 +438,31 @@ struct Server {
 
   static int rows_per_item(const ServerMat& sm, int k, int W) {
     return rows_item(sm.avg_row, lanes_of(sm, W), k, W);
+  }
+
+  // Units (rows, chunks or block rows) of one work item of request p; *units
+  // = the request's units in all. A sliced item is passes of 64 / C chunks of
+  // ~kItemIters slots per lane; a BSR item block rows of ~kItemIters loads per
+  // lane (one block value and one X value per group of 16 right-hand sides).
+  int item_units(const Pending& p, int64_t* units) const {
+    const ServerMat& sm = mats[p.mat];
+    const Matrix& A = *sm.A;
+    const int kind = request_kind(sm, p.k);
+    if (kind == 1) {
+      const auto& L = *static_cast<const SlicedData*>(A.fmt[sm.sell_fmt].get());
+      *units = L.nchunks;
+      const double width = L.nchunks ? (double)L.slots / ((double)L.nchunks * L.C) : 1.0;
+      const int passes = (int)std::clamp(std::ceil(kItemIters / std::max(width, 1.0)), 1.0, (double)kMaxPasses);
+      return passes * (64 / L.C);
+    }
+    if (kind == 2) {
+      const auto& B = *static_cast<const BsrData*>(A.fmt[LESPMV_BSR].get());
+      *units = B.mb;
+      const double per_brow = B.mb ? (double)B.nnzb / (double)B.mb * (1.0 + (double)((p.k + 15) / 16)) : 1.0;
+      return (int)std::clamp(std::ceil(kItemIters / std::max(per_brow, 1.0)), 1.0, (double)kMaxPasses);
+    }
+    *units = A.m;
+    return rows_per_item(sm, p.k, kModeW[rhs_mode(p)]);
   }
 
   // retire finished slots; launch queued requests on free slots
@@ -368,6 +500,153 @@ namespace {
 Server& srv(lespmv_server_t h) {
   LSP_REQUIRE(h, LESPMV_ERR_ARG, "null server");
   return *reinterpret_cast<Server*>(h);
+}
+
+// A layout of a resident matrix: released (its bytes leave the budget), or
+// counted once it is built, after making room for it (the matrix itself is
+// held meanwhile, so it is not the one evicted).
+void release_layout(Server& S, ServerMat& sm, int fmt) {
+  auto& f = sm.A->fmt[fmt];
+  if (!f) return;
+  const int64_t b = f->device_bytes();
+  LSP_REQUIRE(lespmv_release_format(sm.h, fmt) == LESPMV_OK, LESPMV_ERR_DEVICE, "releasing a layout failed");
+  sm.bytes -= b;
+  S.used -= b;
+}
+
+void count_layout(Server& S, ServerMat& sm, int fmt) {
+  const int64_t b = sm.A->fmt[fmt]->device_bytes();
+  sm.pending++;
+  const bool ok = S.make_room(b);
+  sm.pending--;
+  if (!ok) {
+    (void)lespmv_release_format(sm.h, fmt);
+    LSP_REQUIRE(false, LESPMV_ERR_OOM, "no room for the layout: every other resident matrix has requests in flight");
+  }
+  sm.bytes += b;
+  S.used += b;
+}
+
+// k = 1 requests through a sliced layout (S-ELL or SELL-C-sigma, C 32 or 64,
+// every chunk at most kUnitSlots wide), requests with k >= k_from through 4x4
+// BSR (FP64), or (LESPMV_CSR) everything through the row engine.
+void use_format(Server& S, ServerMat& sm, int fmt, const lespmv_params_t* prm, int k_from) {
+  if (fmt == LESPMV_CSR) {
+    release_layout(S, sm, LESPMV_SELL);
+    release_layout(S, sm, LESPMV_SELL_C_SIGMA);
+    release_layout(S, sm, LESPMV_BSR);
+    sm.sell_fmt = -1;
+    sm.bsr_from_k = 0;
+    return;
+  }
+  if (fmt == LESPMV_SELL || fmt == LESPMV_SELL_C_SIGMA) {
+    LSP_REQUIRE(prm && (prm->chunk == 32 || prm->chunk == 64), LESPMV_ERR_ARG,
+                "the serving kernel takes sliced chunks of 32 or 64 rows");
+    release_layout(S, sm, LESPMV_SELL);
+    release_layout(S, sm, LESPMV_SELL_C_SIGMA);
+    sm.sell_fmt = -1;
+    lespmv_params_t p = *prm;
+    if (p.alignment <= 0) p.alignment = 1;
+    const int rc = lespmv_convert(sm.h, fmt, &p);
+    LSP_REQUIRE(rc == LESPMV_OK, rc, "building the sliced layout failed");
+    const auto& L = *static_cast<const SlicedData*>(sm.A->fmt[fmt].get());
+    std::vector<int> width(std::max<int64_t>(L.nchunks, 1), 0);
+    if (L.nchunks) LSP_HIP(hipMemcpy(width.data(), L.chunk_len.p, sizeof(int) * L.nchunks, hipMemcpyDeviceToHost));
+    if (*std::max_element(width.begin(), width.end()) > kUnitSlots) {
+      (void)lespmv_release_format(sm.h, fmt);
+      LSP_REQUIRE(false, LESPMV_ERR_UNSUPPORTED, "a chunk is wider than the serving kernel's 64 slots");
+    }
+    count_layout(S, sm, fmt);
+    sm.sell_fmt = fmt;
+    return;
+  }
+  LSP_REQUIRE(fmt == LESPMV_BSR, LESPMV_ERR_ARG, "the server runs CSR, S-ELL, SELL-C-sigma or BSR");
+  LSP_REQUIRE(S.dtype == LESPMV_F64, LESPMV_ERR_UNSUPPORTED, "BSR requests run on the FP64 matrix cores");
+  LSP_REQUIRE(k_from >= 5 && k_from <= 32, LESPMV_ERR_ARG, "BSR serves k from 5..32 up");
+  release_layout(S, sm, LESPMV_BSR);
+  sm.bsr_from_k = 0;
+  lespmv_params_t p{};
+  p.block_rows = 4;
+  p.block_cols = 4;
+  p.alignment = 1;
+  p.max_fill = prm ? prm->max_fill : 0;
+  const int rc = lespmv_convert(sm.h, LESPMV_BSR, &p);
+  LSP_REQUIRE(rc == LESPMV_OK, rc, "building the BSR layout failed");
+  count_layout(S, sm, LESPMV_BSR);
+  sm.bsr_from_k = k_from;
+}
+
+// Sliced layouts for k = 1: S-ELL and SELL-C-4096 with C in {32, 64}, timed
+// per call against the fastest CSR vector candidate (csr_ms); the fastest is
+// kept when it is at least 3 % faster. Only for matrices whose rows fit the
+// serving kernel's 64-slot chunks.
+void tune_sliced(Server& S, ServerMat& sm, int max_row, double csr_ms) {
+  if (sm.A->nnz == 0 || max_row > kUnitSlots || !(csr_ms < 1e300)) return;
+  std::vector<lespmv_tune_result_t> cand;
+  for (int C : {32, 64})
+    for (int fmt : {LESPMV_SELL, LESPMV_SELL_C_SIGMA}) {
+      lespmv_tune_result_t r{};
+      r.fmt = fmt;
+      r.variant = LESPMV_VARIANT_AUTO;
+      r.params.chunk = C;
+      r.params.sigma = 4096;
+      r.params.alignment = 1;
+      r.params.max_fill = 4;
+      cand.push_back(r);
+    }
+  if (lespmv_tune(sm.h, cand.data(), (int32_t)cand.size(), 5, 1) != LESPMV_OK) return;
+  const lespmv_tune_result_t* best = nullptr;
+  for (const auto& c : cand)
+    if (c.status == LESPMV_OK && (!best || c.ms < best->ms)) best = &c;
+  if (!best || !(best->ms < 0.97 * csr_ms)) return;
+  try {
+    use_format(S, sm, best->fmt, &best->params, 0);
+  } catch (const Error&) {  // no room: the row engine serves k = 1
+  }
+}
+
+// 4x4 BSR on the matrix cores (FP64, block fill at most 1.5 stored values
+// per nonzero): timed per call at k = 8, 16 and 32 against the row engine at
+// its lanes; BSR takes the requests from k = 5, 9 or 17 up when it is at
+// least 3 % faster at 8, 16 and 32, at 16 and 32, or at 32.
+void tune_bsr(Server& S, ServerMat& sm, int64_t n) {
+  Matrix& A = *sm.A;
+  if (S.dtype != LESPMV_F64 || A.nnz == 0) return;
+  lespmv_params_t p{};
+  p.block_rows = 4;
+  p.block_cols = 4;
+  p.alignment = 1;
+  p.max_fill = 2;
+  if (lespmv_convert(sm.h, LESPMV_BSR, &p) != LESPMV_OK) return;
+  const auto& B = *static_cast<const BsrData*>(A.fmt[LESPMV_BSR].get());
+  int from = 0;
+  if ((double)B.nnzb * 16.0 <= 1.5 * (double)A.nnz) {
+    DBuf X, Y;
+    X.alloc(sizeof(double) * n * 32);
+    Y.alloc(sizeof(double) * A.m * 32);
+    LSP_HIP(hipMemset(X.p, 0, sizeof(double) * n * 32));
+    const int ks[3] = {8, 16, 32}, thr[3] = {5, 9, 17};
+    int lg = 0;
+    while ((1 << lg) < sm.lanes_w[2]) lg++;
+    for (int i = 2; i >= 0; i--) {
+      double tb = 0, tc = 0;
+      const int k = ks[i];
+      int rc = lespmv_time_spmm(sm.h, LESPMV_BSR, LESPMV_VARIANT_AUTO, k, 1.0, X.p, 32, 0.0, Y.p, k, 2, &tb);
+      if (rc == LESPMV_OK) rc = lespmv_time_spmm(sm.h, LESPMV_BSR, LESPMV_VARIANT_AUTO, k, 1.0, X.p, 32, 0.0, Y.p, k, 5, &tb);
+      if (rc == LESPMV_OK) rc = lespmv_time_spmm(sm.h, LESPMV_CSR, 16 + lg, k, 1.0, X.p, 32, 0.0, Y.p, k, 5, &tc);
+      if (rc != LESPMV_OK || !(tb < 0.97 * tc)) break;
+      from = thr[i];
+    }
+  }
+  if (from == 0) {
+    (void)lespmv_release_format(sm.h, LESPMV_BSR);
+    return;
+  }
+  try {
+    count_layout(S, sm, LESPMV_BSR);
+    sm.bsr_from_k = from;
+  } catch (const Error&) {  // no room: the row engine serves every k
+  }
 }
 
 // SpMM lanes per row: the row engine timed through lespmv_spmm at k = 2
@@ -448,6 +727,7 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
     sm.h = mh;
     sm.avg_row = (double)nnz / (double)m;
     const bool tune = lanes == 0;
+    double csr_ms = 1e300;
     if (lanes == 0) {  // the autotuner's lanes per row (rows per wave = 64 / lanes) for this matrix
       int32_t count = 0;
       lespmv_tune_candidates(1 << LESPMV_CSR, S.dtype, nullptr, 0, &count);
@@ -455,10 +735,9 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
       lespmv_tune_candidates(1 << LESPMV_CSR, S.dtype, cand.data(), count, &count);
       rc = lespmv_tune(mh, cand.data(), count, 5, 1);
       lanes = 8;
-      double best = 1e300;
       for (const auto& c : cand)
-        if (rc == LESPMV_OK && c.status == LESPMV_OK && c.variant >= 17 && c.variant <= 22 && c.ms < best) {
-          best = c.ms;
+        if (rc == LESPMV_OK && c.status == LESPMV_OK && c.variant >= 17 && c.variant <= 22 && c.ms < csr_ms) {
+          csr_ms = c.ms;
           lanes = 1 << (c.variant - 16);
         }
     }
@@ -472,6 +751,12 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
     // ids are never reused, so a stale id of an evicted matrix stays an error
     S.mats.push_back(std::move(sm));
     *id = (int32_t)S.mats.size() - 1;
+    if (tune) {  // the other layouts, each kept only where it timed faster
+      int max_row = 0;
+      for (int64_t i = 0; i < m; i++) max_row = std::max(max_row, rp[i + 1] - rp[i]);
+      tune_sliced(S, S.mats.back(), max_row, csr_ms);
+      tune_bsr(S, S.mats.back(), n);
+    }
   });
 }
 
@@ -499,6 +784,28 @@ extern "C" int lespmv_server_lanes(lespmv_server_t h, int32_t id, int32_t k, int
     LSP_REQUIRE(id >= 0 && id < (int)S.mats.size() && S.mats[id].live, LESPMV_ERR_ARG, "matrix not resident");
     LSP_REQUIRE(lanes && k >= 1 && k <= 32, LESPMV_ERR_ARG, "bad arguments");
     *lanes = Server::lanes_of(S.mats[id], k == 1 ? 1 : k <= 3 ? 2 : 4);
+  });
+}
+
+extern "C" int lespmv_server_use(lespmv_server_t h, int32_t id, int32_t fmt, const lespmv_params_t* params,
+                                 int32_t k_from) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(id >= 0 && id < (int)S.mats.size() && S.mats[id].live, LESPMV_ERR_ARG, "matrix not resident");
+    LSP_REQUIRE(S.mats[id].pending == 0, LESPMV_ERR_ARG, "matrix has requests in flight");
+    use_format(S, S.mats[id], fmt, params, k_from);
+    S.mats[id].last_use = ++S.tick;
+  });
+}
+
+extern "C" int lespmv_server_format(lespmv_server_t h, int32_t id, int32_t k, int32_t* fmt) {
+  return guarded_call([&] {
+    Server& S = srv(h);
+    LSP_REQUIRE(id >= 0 && id < (int)S.mats.size() && S.mats[id].live, LESPMV_ERR_ARG, "matrix not resident");
+    LSP_REQUIRE(fmt && k >= 1 && k <= 32, LESPMV_ERR_ARG, "bad arguments");
+    const ServerMat& sm = S.mats[id];
+    const int kind = request_kind(sm, k);
+    *fmt = kind == 1 ? sm.sell_fmt : kind == 2 ? LESPMV_BSR : LESPMV_CSR;
   });
 }
 

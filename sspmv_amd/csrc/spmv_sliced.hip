@@ -15,6 +15,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "sliced_engine.hpp"
 
 namespace lsp {
 namespace {
@@ -76,7 +77,6 @@ __global__ void sliced_fill(int64_t m, int C, const int* __restrict__ rp, const 
 // alone; a unit that covers a whole chunk writes y, the others write partials
 // that sliced_fixup adds in unit order. One thread per (unit, row of chunk);
 // padding sits at the end of each row, so a col < 0 ends the row.
-constexpr int kUnitSlots = 64;
 // Column and value slots are read once, with non-temporal loads, so they do
 // not push x out of the caches (27-point stencil 160^3 FP64 278 -> 263 us,
 // FP32 170 -> 154 us; lognormal rows 905 -> 887 us; 2-D Laplacian 1024^2
@@ -106,42 +106,7 @@ __global__ __launch_bounds__(kBlock) void sliced_spmv_kernel(int64_t m, int64_t 
   const int4 un = units[u];  // chunk, j0, j1, partial slot (-1: whole chunk)
   const int64_t row = (int64_t)un.x * C + r;
   if (row >= m) return;
-  const int* cp = col + coff[un.x] + r;
-  const V* vp = sv + coff[un.x] + r;
-  V sum = 0;
-  // kUnroll slots per step; the next step's column and value loads are
-  // issued before this step's gathers are used, so they overlap
-  int c[kUnroll];
-  V v[kUnroll];
-#pragma unroll
-  for (int q = 0; q < kUnroll; q++) {
-    const bool ok = un.y + q < un.z;
-    c[q] = ok ? __builtin_nontemporal_load(cp + (int64_t)(un.y + q) * C) : -1;
-    v[q] = ok ? __builtin_nontemporal_load(vp + (int64_t)(un.y + q) * C) : V(0);
-  }
-  for (int j = un.y; j < un.z; j += kUnroll) {
-    V xv[kUnroll];
-#pragma unroll
-    for (int q = 0; q < kUnroll; q++) xv[q] = c[q] >= 0 ? x[c[q]] : V(0);
-    const bool more = c[kUnroll - 1] >= 0 && j + kUnroll < un.z;
-    int cn[kUnroll];
-    V vn[kUnroll];
-#pragma unroll
-    for (int q = 0; q < kUnroll; q++) {
-      const bool ok = more && j + kUnroll + q < un.z;
-      cn[q] = ok ? __builtin_nontemporal_load(cp + (int64_t)(j + kUnroll + q) * C) : -1;
-      vn[q] = ok ? __builtin_nontemporal_load(vp + (int64_t)(j + kUnroll + q) * C) : V(0);
-    }
-#pragma unroll
-    for (int q = 0; q < kUnroll; q++)
-      if (c[q] >= 0) sum += v[q] * xv[q];
-    if (!more) break;
-#pragma unroll
-    for (int q = 0; q < kUnroll; q++) {
-      c[q] = cn[q];
-      v[q] = vn[q];
-    }
-  }
+  const V sum = sliced_row_sum<V, kUnroll>(col + coff[un.x] + r, sv + coff[un.x] + r, C, un.y, un.z, x);
   if (un.w < 0) {
     const int64_t out = order ? order[row] : row;
     y[out] = beta == V(0) ? alpha * sum : alpha * sum + beta * y[out];

@@ -182,6 +182,8 @@ def load() -> C.CDLL:
         "lespmv_server_remove": ([vp, i32], C.c_int),
         "lespmv_server_matrix": ([vp, i32, C.POINTER(vp), C.POINTER(i32)], C.c_int),
         "lespmv_server_lanes": ([vp, i32, i32, C.POINTER(i32)], C.c_int),
+        "lespmv_server_use": ([vp, i32, i32, vp, i32], C.c_int),
+        "lespmv_server_format": ([vp, i32, i32, C.POINTER(i32)], C.c_int),
         "lespmv_server_submit": ([vp, i32, i32, dbl, vp, i64, dbl, vp, i64, C.POINTER(i64)], C.c_int),
         "lespmv_server_poll": ([vp, i32, vp, vp, i32, C.POINTER(i32)], C.c_int),
         "lespmv_server_drain": ([vp], C.c_int),
@@ -356,9 +358,17 @@ class Matrix:
     def from_host(cls, A) -> "Matrix":
         return cls(A.m, A.n, A.rp, A.ci, A.v)
 
+    @classmethod
+    def borrowed(cls, h, m: int, n: int, nnz: int, dtype, owner=None) -> "Matrix":
+        """A handle owned elsewhere (a server's resident matrix); not destroyed here."""
+        M = cls.__new__(cls)
+        M.h, M.dtype, M.m, M.n, M.nnz = h, np.dtype(dtype), int(m), int(n), int(nnz)
+        M._owner = owner  # keeps the owner alive while the handle is in use
+        return M
+
     def __del__(self):
         try:
-            if self.h:
+            if self.h and getattr(self, "_owner", None) is None:
                 load().lespmv_destroy(self.h)
                 self.h = C.c_void_p()
         except Exception:
@@ -619,6 +629,22 @@ class Server:
         lanes = C.c_int32()
         check(load().lespmv_server_lanes(self.h, mid, k, C.byref(lanes)))
         return lanes.value
+
+    def use(self, mid: int, fmt: int, params: "Params | None" = None, k_from: int = 0):
+        """Serve the matrix from a layout: a sliced format for k = 1, BSR 4x4 for k >= k_from, or CSR only."""
+        check(load().lespmv_server_use(self.h, mid, fmt, C.byref(params) if params is not None else None, k_from))
+
+    def format(self, mid: int, k: int = 1) -> int:
+        """The format this matrix's requests with k right-hand sides run on."""
+        f = C.c_int32()
+        check(load().lespmv_server_format(self.h, mid, k, C.byref(f)))
+        return f.value
+
+    def matrix(self, mid: int, A) -> "Matrix":
+        """The server's handle of resident matrix `mid` (host matrix A gives its shape), borrowed."""
+        h = C.c_void_p()
+        check(load().lespmv_server_matrix(self.h, mid, C.byref(h), None))
+        return Matrix.borrowed(h, A.m, A.n, len(A.ci), self.dtype, owner=self)
 
     def submit(self, mid: int, k: int, alpha: float, X: "DeviceArray", ldx: int, beta: float, Y: "DeviceArray",
                ldy: int) -> int:

@@ -1,7 +1,8 @@
 """Batched serving on the GPU (lespmv_server_*): the packed persistent kernel
-replayed from a HIP graph against per-call CSR SpMM and the oracle, results
-identical alone and in a mix, repeated graph replays with other work in
-between, and admission / eviction by HBM footprint."""
+replayed from a HIP graph against the per-call kernel of the format each
+request runs on (CSR row engine, sliced layouts, BSR 4x4 on the matrix cores)
+and the oracle, results identical alone and in a mix, repeated graph replays
+with other work in between, and admission / eviction by HBM footprint."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,26 @@ def run(lib, S, mids, As, reqs):
     return [dY.numpy() for _, dY in bufs]
 
 
+def percall(lib, S, mid, A, k, alpha, dX, ldx, beta, Y0, ldy):
+    """The request's product from the per-call kernel of the format the server
+    runs it on (which gives bitwise the server's result), or None where that
+    kernel cannot take the request's layout (a sliced k = 1 request whose x or y
+    is strided)."""
+    fmt = S.format(mid, k)
+    dY = lib.DeviceArray.from_numpy(Y0)
+    if fmt == lib.CSR:
+        M = lib.Matrix.from_host(A)
+        M.spmm(lib.CSR, k, alpha, dX, ldx, beta, dY, ldy, variant=16 + S.lanes(mid, k).bit_length() - 1)
+    elif fmt == lib.BSR:
+        S.matrix(mid, A).spmm(lib.BSR, k, alpha, dX, ldx, beta, dY, ldy)
+    else:
+        assert fmt in (lib.SELL, lib.SELL_C_SIGMA) and k == 1
+        if ldx != 1 or ldy != 1:
+            return None
+        S.matrix(mid, A).spmv(fmt, alpha, dX, beta, dY)
+    return dY.numpy()
+
+
 REQS = [(0, 1, 1.0, 0.0, 10), (1, 3, 0.8, 0.7, 11), (2, 8, 1.0, 0.0, 12), (3, 32, -0.5, 2.0, 13),
         (4, 5, 1.0, 0.0, 14), (1, 1, 1.0, 0.0, 15), (3, 17, 1.0, 0.0, 16), (0, 32, 0.3, 0.0, 17)]
 
@@ -47,6 +68,10 @@ def test_packed_kernel_matches_percall_and_oracle(gpu, oracle):
     S = lib.Server(np.float64, 1 << 30, max_batch=64, slots=2)
     mids = [S.add(A) for A in As]
     assert all(S.lanes(m) in (2, 4, 8, 16, 32, 64) for m in mids)
+    allowed = {1: (lib.CSR, lib.SELL, lib.SELL_C_SIGMA)}
+    for m in mids:
+        for k in (1, 4, 5, 9, 16, 17, 32):
+            assert S.format(m, k) in allowed.get(k, (lib.CSR, lib.BSR)), (m, k)
     for (i, k, alpha, beta, seed), Y in zip(REQS, run(lib, S, mids, As, REQS)):
         A = As[i]
         X, Y0 = dense_inputs(lib, A, k, seed)
@@ -54,11 +79,8 @@ def test_packed_kernel_matches_percall_and_oracle(gpu, oracle):
         dX, dY = lib.DeviceArray.from_numpy(X), lib.DeviceArray.from_numpy(Y0)
         M.spmm(lib.CSR, k, alpha, dX, k, beta, dY, k)
         assert rel2(Y, dY.numpy()) <= 1e-12, (i, k)
-        # the row engine launched on its own (same lanes) gives bitwise the server's result
-        lanes = S.lanes(mids[i], k)
-        dY2 = lib.DeviceArray.from_numpy(Y0)
-        M.spmm(lib.CSR, k, alpha, dX, k, beta, dY2, k, variant=16 + lanes.bit_length() - 1)
-        np.testing.assert_array_equal(Y, dY2.numpy())
+        # the per-call kernel of the request's format gives bitwise the server's result
+        np.testing.assert_array_equal(Y, percall(lib, S, mids[i], A, k, alpha, dX, k, beta, Y0, k))
         # column 0 against the oracle's CSR SpMV
         ref = oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v, alpha, np.ascontiguousarray(X[:, 0]), beta,
                               np.ascontiguousarray(Y0[:, 0]))
@@ -83,9 +105,9 @@ def test_same_result_alone_and_in_mix(gpu):
     alone2 = [run(lib, S2, mids2, As, [r])[0] for r in REQS]
     for a, b in zip(alone2, run(lib, S2, mids2, As, REQS)):
         np.testing.assert_array_equal(a, b)
-    # k = 1 requests use the same lanes on both servers
+    # k = 1 requests use the same lanes on both servers (where the first runs them on CSR too)
     for (i, k, *_), a, b in zip(REQS, alone, alone2):
-        if k == 1:
+        if k == 1 and S.format(mids[i], 1) == lib.CSR:
             np.testing.assert_array_equal(a, b)
     assert S2.stats()["batches"] >= 3
 
@@ -164,13 +186,103 @@ def test_fp32_server_and_padded_leading_dimensions(gpu, oracle):
         for (i, k, ldx, ldy), (X, Y0, dX, dY) in zip(cases, bufs):
             A = As[i]
             Y = dY.numpy()
-            M = lib.Matrix.from_host(A)
-            dY2 = lib.DeviceArray.from_numpy(Y0)
-            M.spmm(lib.CSR, k, 0.9, dX, ldx, 0.5, dY2, ldy, variant=16 + S.lanes(mids[i], k).bit_length() - 1)
-            np.testing.assert_array_equal(Y, dY2.numpy())
+            same = percall(lib, S, mids[i], A, k, 0.9, dX, ldx, 0.5, Y0, ldy)
+            if same is not None:
+                np.testing.assert_array_equal(Y, same)
             np.testing.assert_array_equal(Y[:, k:], Y0[:, k:])
             for j in range(k):
                 ref = Y0[:, j].astype(np.float64)
                 oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v.astype(np.float64), 0.9, X[:, j].astype(np.float64), 0.5,
                                 ref)
                 assert rel2(Y[:, j], ref) <= TOL[np.dtype(dt)], (np.dtype(dt).name, i, k, j)
+
+
+def test_sliced_and_bsr_requests(gpu, oracle):
+    """Resident matrices set to sliced layouts (k = 1; S-ELL and SELL-C-sigma,
+    C 32 and 64) and to 4x4 BSR on the matrix cores (k >= 5) by
+    lespmv_server_use: every request equals the per-call kernel of its format
+    bitwise, is the same alone and in the mix, and is within 1e-12 (FP64) or
+    1e-5 (FP32) of the FP64 oracle in every column, strided x included; the
+    layouts' bytes count against the budget, and the refusals give their codes."""
+    lib = gpu
+    rng = np.random.default_rng(5)
+    for dt in (np.float64, np.float32):
+        As = [lib.generate(lib.GEN_LAPLACE2D, [60, 50, 4.0, -1.0], 0, dt),
+              lib.generate(lib.GEN_STENCIL27, [14, 26.0, -1.0], 5, dt),
+              lib.generate(lib.GEN_BLOCKRAND, [400, 380, 4, 4, 8], 3, dt),
+              lib.generate(lib.GEN_POWERLAW, [5000, 2.0, 8.0, 2000], 4, dt)]
+        S = lib.Server(dt, 1 << 30, max_batch=64, slots=2)
+        mids = [S.add(A, lanes=4) for A in As]
+        csr_bytes = S.stats()["bytes_used"]
+        assert all(S.format(m, k) == lib.CSR for m in mids for k in (1, 8, 32))
+        S.use(mids[0], lib.SELL_C_SIGMA, lib.make_params(chunk=64, sigma=256, alignment=1))
+        S.use(mids[1], lib.SELL, lib.make_params(chunk=32, alignment=1))
+        S.use(mids[2], lib.SELL_C_SIGMA, lib.make_params(chunk=32, sigma=4096, alignment=1))
+        with pytest.raises(lib.LespmvError) as e:  # rows of up to 2000 nonzeros
+            S.use(mids[3], lib.SELL, lib.make_params(chunk=64, alignment=1))
+        assert e.value.code == lib.ERR_UNSUPPORTED and S.format(mids[3], 1) == lib.CSR
+        with pytest.raises(lib.LespmvError) as e:
+            S.use(mids[0], lib.SELL, lib.make_params(chunk=16, alignment=1))
+        assert e.value.code == lib.ERR_ARG
+        if dt == np.float64:
+            S.use(mids[2], lib.BSR, None, 5)
+            with pytest.raises(lib.LespmvError) as e:
+                S.use(mids[1], lib.BSR, None, 4)
+            assert e.value.code == lib.ERR_ARG
+        else:
+            with pytest.raises(lib.LespmvError) as e:
+                S.use(mids[2], lib.BSR, None, 5)
+            assert e.value.code == lib.ERR_UNSUPPORTED
+        assert S.stats()["bytes_used"] > csr_bytes
+        assert S.format(mids[0], 1) == lib.SELL_C_SIGMA and S.format(mids[1], 1) == lib.SELL
+        assert S.format(mids[0], 2) == lib.CSR
+        bsr = lib.BSR if dt == np.float64 else lib.CSR
+        assert S.format(mids[2], 4) == lib.CSR and S.format(mids[2], 5) == bsr and S.format(mids[2], 32) == bsr
+        # (matrix, k, ldx, ldy, alpha, beta)
+        reqs = [(0, 1, 1, 1, 1.0, 0.0), (1, 1, 1, 1, 0.8, 0.7), (2, 1, 1, 1, -1.0, 0.5), (3, 1, 1, 1, 1.0, 0.0),
+                (0, 1, 3, 2, 0.5, 1.5), (1, 1, 4, 1, 1.0, 0.0), (2, 8, 8, 8, 1.0, 0.0), (2, 16, 16, 17, 0.7, -0.3),
+                (2, 32, 32, 32, 1.0, 0.0), (2, 5, 7, 5, 2.0, 0.0), (2, 21, 21, 24, 1.0, 1.0), (2, 4, 4, 4, 1.0, 0.0),
+                (1, 9, 9, 9, 1.0, 0.0), (0, 3, 3, 3, 1.0, 0.0)]
+        data = []
+        for i, k, ldx, ldy, a, b in reqs:
+            A = As[i]
+            X = rng.uniform(-1, 1, (A.n, ldx)).astype(dt)
+            Y0 = rng.uniform(-1, 1, (A.m, ldy)).astype(dt)
+            data.append((X, Y0))
+
+        def serve(which):
+            out = []
+            for r in which:
+                i, k, ldx, ldy, a, b = reqs[r]
+                X, Y0 = data[r]
+                dX, dY = lib.DeviceArray.from_numpy(X), lib.DeviceArray.from_numpy(Y0)
+                S.submit(mids[i], k, a, dX, ldx, b, dY, ldy)
+                out.append((dX, dY))
+            return out
+
+        mixed = serve(range(len(reqs)))
+        S.drain()
+        S.poll()
+        tol = TOL[np.dtype(dt)]
+        for r, (dX, dY) in enumerate(mixed):
+            i, k, ldx, ldy, a, b = reqs[r]
+            A = As[i]
+            X, Y0 = data[r]
+            Y = dY.numpy()
+            alone = serve([r])
+            S.drain()
+            S.poll()
+            np.testing.assert_array_equal(Y, alone[0][1].numpy(), err_msg=f"request {r}")
+            same = percall(lib, S, mids[i], A, k, a, dX, ldx, b, Y0, ldy)
+            if same is not None:
+                np.testing.assert_array_equal(Y, same, err_msg=f"request {r}")
+            np.testing.assert_array_equal(Y[:, k:], Y0[:, k:])
+            for j in range(k):
+                ref = Y0[:, j].astype(np.float64)
+                oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v.astype(np.float64), a, X[:, j].astype(np.float64), b, ref)
+                assert rel2(Y[:, j], ref) <= tol, (r, j)
+        # back to the row engine only: the layouts leave the budget
+        for m in mids:
+            S.use(m, lib.CSR)
+        assert S.stats()["bytes_used"] == csr_bytes
+        assert all(S.format(m, k) == lib.CSR for m in mids for k in (1, 8, 32))
