@@ -13,10 +13,13 @@
 //  * vector (variant 1): one thread per row, x through L1/L2;
 //  * window (variant 2): a workgroup owns 256 rows and stages the x windows it
 //    needs in LDS, one window per cluster of nearby offsets (a 27-point stencil
-//    has three clusters); falls back to variant 1 when the windows do not fit.
+//    has three clusters); falls back to variant 1 when the windows do not fit;
+//  * SpMM: the DIA row engine (dia_engine.hpp): W right-hand sides per lane
+//    and the loads of 8 diagonals in flight.
 #include <algorithm>
 
 #include "common.hpp"
+#include "dia_engine.hpp"
 
 namespace lsp {
 namespace {
@@ -193,22 +196,17 @@ __global__ __launch_bounds__(kBlock) void dia_window(int64_t m, int64_t n, int n
   y[i] = beta == V(0) ? alpha * sum : alpha * sum + beta * y[i];
 }
 
-template <class V, int KP>
-__global__ __launch_bounds__(kBlock) void dia_spmm_kernel(int64_t m, int64_t n, int k, int nd, int64_t stride,
-                                                          const int64_t* __restrict__ offs,
-                                                          const V* __restrict__ data, const V* __restrict__ X,
-                                                          int64_t ldx, V* __restrict__ Y, int64_t ldy, V alpha,
-                                                          V beta) {
-  const int c = threadIdx.x & (KP - 1);
-  const int64_t i = (int64_t)blockIdx.x * (kBlock / KP) + threadIdx.x / KP;
-  if (i >= m || c >= k) return;
-  V sum = 0;
-  for (int d = 0; d < nd; d++) {
-    const int64_t j = i + offs[d];
-    if (j >= 0 && j < n) sum += data[(int64_t)d * stride + i] * X[j * ldx + c];
-  }
-  V* yp = Y + i * ldy + c;
-  *yp = beta == V(0) ? alpha * sum : alpha * sum + beta * *yp;
+template <class V, int W, bool VEC>
+__global__ __launch_bounds__(kBlock) void dia_rows_kernel(const int64_t* __restrict__ offs,
+                                                          const V* __restrict__ data, int64_t stride, int nd,
+                                                          int64_t m, int64_t n, int k, const V* __restrict__ X,
+                                                          int64_t ldx, V* __restrict__ Y, int64_t ldy, double alpha,
+                                                          double beta, int rpi, int64_t nitems) {
+  const int64_t item = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + threadIdx.x / 64;
+  if (item >= nitems) return;  // whole waves leave together
+  const int r0 = (int)(item * rpi);
+  const int r1 = r0 + rpi < m ? r0 + rpi : (int)m;
+  dia_rows<V, W, VEC>(offs, data, stride, nd, n, k, X, ldx, Y, ldy, alpha, beta, r0, r1, threadIdx.x & 63);
 }
 
 struct DiaDev : DiaData {
@@ -294,25 +292,29 @@ void run(Matrix& A, DiaDev& d, int variant, double alpha, const void* x, double 
   LSP_LAUNCHED();
 }
 
-template <class V, int KP>
-void spmm_launch(Matrix& A, DiaDev& d, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
-                 int64_t ldy, hipStream_t s) {
-  hipLaunchKernelGGL((dia_spmm_kernel<V, KP>), grid1d(A.m, kBlock / KP), dim3(kBlock), 0, s, A.m, A.n, k, d.ndiag,
-                     d.stride, d.offsets.as<int64_t>(), d.data.as<V>(), static_cast<const V*>(X), ldx,
-                     static_cast<V*>(Y), ldy, (V)alpha, (V)beta);
-  LSP_LAUNCHED();
-}
+// One pass of rows per work item (a wave): the DIA rows carry all their loads
+// in one pass already.
+constexpr int kDiaCallIters = 8;
 
 template <class V>
 void spmm_dispatch(Matrix& A, DiaDev& d, int k, double alpha, const void* X, int64_t ldx, double beta, void* Y,
                    int64_t ldy, hipStream_t s) {
-  if (k <= 1) spmm_launch<V, 1>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
-  else if (k <= 2) spmm_launch<V, 2>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
-  else if (k <= 4) spmm_launch<V, 4>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
-  else if (k <= 8) spmm_launch<V, 8>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
-  else if (k <= 16) spmm_launch<V, 16>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
-  else if (k <= 32) spmm_launch<V, 32>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
-  else spmm_launch<V, 64>(A, d, k, alpha, X, ldx, beta, Y, ldy, s);
+  const int mode = rows_mode(k, ldx, ldy, X, Y, (int64_t)sizeof(V));
+  const int rpi = dia_item_rows(d.ndiag, k, kModeW[mode], kDiaCallIters);
+  const int64_t nitems = ceil_div(A.m, rpi);
+  void (*kern)(const int64_t*, const V*, int64_t, int, int64_t, int64_t, int, const V*, int64_t, V*, int64_t, double,
+               double, int, int64_t);
+  switch (mode) {
+    case 0: kern = dia_rows_kernel<V, 1, false>; break;
+    case 1: kern = dia_rows_kernel<V, 2, false>; break;
+    case 2: kern = dia_rows_kernel<V, 2, true>; break;
+    case 3: kern = dia_rows_kernel<V, 4, false>; break;
+    default: kern = dia_rows_kernel<V, 4, true>; break;
+  }
+  hipLaunchKernelGGL(kern, grid1d(nitems, kBlock / 64), dim3(kBlock), 0, s, d.offsets.as<int64_t>(), d.data.as<V>(),
+                     d.stride, d.ndiag, A.m, A.n, k, static_cast<const V*>(X), ldx, static_cast<V*>(Y), ldy, alpha,
+                     beta, rpi, nitems);
+  LSP_LAUNCHED();
 }
 
 }  // namespace
