@@ -98,7 +98,10 @@ __device__ __forceinline__ void load_w_buf(__amdgpu_buffer_rsrc_t r, uint32_t of
   }
 }
 
-// Rows [r0, r1) of request q for all k right-hand sides. A row gets G lanes
+// Rows [r0, r1) of request q for all k right-hand sides (positions [r0, r1)
+// of q.order when q.order is set: rows sorted by length within windows, so
+// that the rows of a pass need about the same number of steps). A row gets G
+// lanes
 // (G = max(lanes, kk), kk = lanes needed to cover k at W right-hand sides
 // each, rounded up to a power of two); lane t handles right-hand sides
 // (t mod kk)*W .. +W-1 and every (G/kk)-th nonzero of the row, and the
@@ -131,9 +134,13 @@ __device__ __forceinline__ void run_rows(const Q& q, int r0, int r1, int lane) {
     vr = make_rsrc(q.val, kMaxBufferBytes);
     xr = make_rsrc(q.X, kMaxBufferBytes);
   }
+  const LSP_GLOBAL_AS int* __restrict__ order = (const LSP_GLOBAL_AS int*)q.order;
   for (int base = r0; base < r1; base += P) {
-    const int row = base + grp;
-    const bool ok = row < r1 && j < k;
+    // position base + grp of the item; with an order, the rows of a pass are
+    // taken from it (a row's sum does not depend on which pass computes it)
+    const int pos = base + grp;
+    const bool ok = pos < r1 && j < k;
+    const int row = ok && order ? order[pos] : pos;
     double acc[W];  // FP64 sums for FP32 data too: products and sums round once, at the store
 #pragma unroll
     for (int u = 0; u < W; u++) acc[u] = 0;

@@ -9,9 +9,11 @@
 // kUnitSlots wide) for k = 1 and a 4x4 BSR layout (FP64, matrix cores) for k
 // from a timed threshold up. A batch is a table of request descriptors (matrix
 // arrays, X, Y, k, alpha, beta, units per work item, first work item); a
-// request's work items are consecutive ranges of rows (CSR), chunks (sliced) or
-// block rows (BSR), sized on the host from the matrix's mean row length and k
-// so that an item is a few passes of ~kItemIters loads per lane. Each request
+// request's work items are consecutive ranges of rows (CSR; positions in a
+// per-matrix order that sorts rows by length within windows of 64 where that
+// balances the row engine's passes), chunks (sliced) or block rows (BSR),
+// sized on the host from the matrix's mean row length and k so that an item
+// is a few passes of ~kItemIters loads per lane. Each request
 // is placed on one XCD (largest work first, least loaded XCD): its items form
 // that XCD's queue, so its X is gathered into one L2.
 // The persistent kernel's waves take a few items at a time (the batch's grab
@@ -206,6 +208,7 @@ struct ServerMat {
   double avg_row = 0.0;
   int32_t lanes = 1;                   // k = 1
   int32_t lanes_w[3] = {1, 1, 1};      // per right-hand sides per lane W = 1, 2, 4
+  DBuf order;                          // row-engine rows sorted by length within windows (empty: natural order)
   int32_t sell_fmt = -1;               // k = 1 through this sliced layout (S-ELL or SELL-C-sigma); -1: row engine
   int32_t bsr_from_k = 0;              // k >= this through the 4x4 BSR layout on the matrix cores; 0: never
   int64_t bytes = 0, last_use = 0, pending = 0;
@@ -322,6 +325,7 @@ struct Server {
     ServerMat& sm = mats[i];
     used -= sm.bytes;
     sm.A.reset();
+    sm.order.reset();
     sm.h = nullptr;
     sm.live = false;
   }
@@ -409,6 +413,7 @@ struct Server {
           q.rp = A.rp.as<int>();
           q.ci = A.ci.as<int>();
           q.val = A.val.p;
+          q.order = sm.order.p ? sm.order.as<int>() : nullptr;
           q.mode = mode[i] | (rows_buffer_ok(A.nnz, A.n, p.ldx, vsize(dtype)) ? kBufferMode : 0);
         }
         R[d++] = q;
@@ -519,6 +524,41 @@ namespace {
 Server& srv(lespmv_server_t h) {
   LSP_REQUIRE(h, LESPMV_ERR_ARG, "null server");
   return *reinterpret_cast<Server*>(h);
+}
+
+// The row engine runs the rows of a pass (64 / G rows side by side) for as
+// many steps as the longest of them needs. Rows sorted by length (longest
+// first, then by row) within windows of kOrderWindow rows make the rows of a
+// pass alike; the order is kept when it cuts the steps of passes of
+// kOrderPass rows by at least 15 % (lognormal rows). A row's sum does not
+// depend on the pass that computes it, so results are unchanged. Serving mix
+// at saturation: 77k req/s in row order, 84k with windows of 64 rows, 83k
+// with 32 or 128, 83k with 256, 82k with 1024, 79k sorted whole (p99 at 20k
+// req/s 0.49, 0.49, 0.50-0.56, 0.59 and 1.0 ms: long windows put the long
+// rows of a matrix into a few work items).
+constexpr int kOrderWindow = 64;
+constexpr int kOrderPass = 8;
+
+std::vector<int> row_order(int64_t m, const int32_t* rp) {
+  auto len = [&](int64_t r) { return rp[r + 1] - rp[r]; };
+  auto steps = [&](auto row_at) {
+    int64_t total = 0;
+    for (int64_t b = 0; b < m; b += kOrderPass) {
+      int mx = 0;
+      for (int64_t i = b; i < std::min<int64_t>(b + kOrderPass, m); i++) mx = std::max(mx, len(row_at(i)));
+      total += mx;
+    }
+    return total;
+  };
+  std::vector<int> ord(m);
+  for (int64_t i = 0; i < m; i++) ord[i] = (int)i;
+  for (int64_t w = 0; w < m; w += kOrderWindow)
+    std::stable_sort(ord.begin() + w, ord.begin() + std::min<int64_t>(w + kOrderWindow, m),
+                     [&](int a, int b) { return len(a) > len(b); });
+  const int64_t natural = steps([](int64_t i) { return i; });
+  const int64_t sorted = steps([&](int64_t i) { return (int64_t)ord[i]; });
+  if ((double)sorted > 0.85 * (double)natural) ord.clear();
+  return ord;
 }
 
 // A layout of a resident matrix: released (its bytes leave the budget), or
@@ -735,7 +775,8 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
     LSP_REQUIRE(lanes == 0 || (lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0), LESPMV_ERR_ARG,
                 "lanes must be 0 (tune) or a power of two <= 64");
     const int64_t vs = vsize(S.dtype);
-    const int64_t bytes = (m + 1) * 4 + nnz * (4 + vs);
+    const std::vector<int> ord = row_order(m, rp);
+    const int64_t bytes = (m + 1) * 4 + nnz * (4 + vs) + (int64_t)ord.size() * 4;
     LSP_REQUIRE(bytes <= S.budget, LESPMV_ERR_OOM, "matrix larger than the server's HBM budget");
     LSP_REQUIRE(S.make_room(bytes), LESPMV_ERR_OOM, "no room: every resident matrix has requests in flight");
     lespmv_matrix_t mh = nullptr;
@@ -744,6 +785,10 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
     ServerMat sm;
     sm.A.reset(reinterpret_cast<Matrix*>(mh));
     sm.h = mh;
+    if (!ord.empty()) {
+      sm.order.alloc(sizeof(int) * ord.size());
+      LSP_HIP(hipMemcpy(sm.order.p, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+    }
     sm.avg_row = (double)nnz / (double)m;
     const bool tune = lanes == 0;
     double csr_ms = 1e300;
