@@ -189,6 +189,10 @@ struct Matrix {
   hipEvent_t ev_switch = nullptr;
   double prepare_ms = 0;  // device time of the last CSR prepare (panel layout)
   DBuf rp, ci, val;  // CSR: int32[m+1], int32[nnz], V[nnz]
+  // int32[m]: the rows the CSR row engine (SpMM variants 16..22) takes, by
+  // position; empty = natural order. Set by the serving layer for matrices
+  // whose rows differ in length; results do not depend on it.
+  DBuf row_order;
   CsrMeta meta;
   std::unique_ptr<FormatData> fmt[LESPMV_NUM_FORMATS];
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -208,7 +208,6 @@ struct ServerMat {
   double avg_row = 0.0;
   int32_t lanes = 1;                   // k = 1
   int32_t lanes_w[3] = {1, 1, 1};      // per right-hand sides per lane W = 1, 2, 4
-  DBuf order;                          // row-engine rows sorted by length within windows (empty: natural order)
   int32_t sell_fmt = -1;               // k = 1 through this sliced layout (S-ELL or SELL-C-sigma); -1: row engine
   int32_t bsr_from_k = 0;              // k >= this through the 4x4 BSR layout on the matrix cores; 0: never
   int64_t bytes = 0, last_use = 0, pending = 0;
@@ -325,7 +324,6 @@ struct Server {
     ServerMat& sm = mats[i];
     used -= sm.bytes;
     sm.A.reset();
-    sm.order.reset();
     sm.h = nullptr;
     sm.live = false;
   }
@@ -413,7 +411,7 @@ struct Server {
           q.rp = A.rp.as<int>();
           q.ci = A.ci.as<int>();
           q.val = A.val.p;
-          q.order = sm.order.p ? sm.order.as<int>() : nullptr;
+          q.order = A.row_order.p ? A.row_order.as<int>() : nullptr;
           q.mode = mode[i] | (rows_buffer_ok(A.nnz, A.n, p.ldx, vsize(dtype)) ? kBufferMode : 0);
         }
         R[d++] = q;
@@ -785,9 +783,9 @@ extern "C" int lespmv_server_add_csr(lespmv_server_t h, int64_t m, int64_t n, in
     ServerMat sm;
     sm.A.reset(reinterpret_cast<Matrix*>(mh));
     sm.h = mh;
-    if (!ord.empty()) {
-      sm.order.alloc(sizeof(int) * ord.size());
-      LSP_HIP(hipMemcpy(sm.order.p, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+    if (!ord.empty()) {  // on the handle, so that the lanes below are timed in this order too
+      sm.A->row_order.alloc(sizeof(int) * ord.size());
+      LSP_HIP(hipMemcpy(sm.A->row_order.p, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
     }
     sm.avg_row = (double)nnz / (double)m;
     const bool tune = lanes == 0;

@@ -761,8 +761,9 @@ void rows_dispatch(Matrix& A, int lanes, int k, double alpha, const void* X, int
   const int mode = rows_mode(k, ldx, ldy, X, Y, (int64_t)sizeof(V));
   const int rpi = rows_item(A.meta.avg_row, lanes, k, kModeW[mode], kCallIters);
   const int64_t nitems = ceil_div(A.m, rpi);
-  const RowsArgs a{A.rp.as<int>(), A.ci.as<int>(), A.val.p, X, Y, nullptr, ldx, ldy, alpha, beta, (int32_t)A.m, k,
-                   lanes, rpi};
+  const RowsArgs a{A.rp.as<int>(), A.ci.as<int>(), A.val.p, X, Y,
+                   A.row_order.p ? A.row_order.as<int>() : nullptr, ldx, ldy, alpha, beta, (int32_t)A.m, k, lanes,
+                   rpi};
   const dim3 grid = grid1d(nitems, kBlock / 64);
   const bool buf = rows_buffer_ok(A.nnz, A.n, ldx, (int64_t)sizeof(V));
   void (*kern)(RowsArgs, int64_t);
