@@ -238,6 +238,10 @@ constexpr int kGraphs = 3;  // persistent grids of three sizes per slot
 // graph launch (single-request latency p50 66 -> 59 us); larger batches
 // replay the captured graph.
 constexpr int kDirectBatch = 4;
+// A batch takes at most ceil(queue / slots) requests, but at least this many:
+// the last batches of a burst then run side by side instead of one large
+// batch alone (serving mix at saturation 83.6k -> 84.6k req/s).
+constexpr int kSpreadMin = 32;
 
 struct Slot {
   hipStream_t s = nullptr;
@@ -334,7 +338,12 @@ struct Server {
     // building leaves every request queued)
     int n = 0;
     int64_t total = 0;
-    while (n < (int)std::min<size_t>(queue.size(), (size_t)max_batch)) {
+    // a short queue is spread over the slots (at least kSpreadMin requests a
+    // batch), so that the last batches of a burst run side by side
+    const size_t cap = std::min<size_t>(
+        {queue.size(), (size_t)max_batch,
+         std::max<size_t>((size_t)kSpreadMin, (queue.size() + slots.size() - 1) / slots.size())});
+    while (n < (int)cap) {
       const Pending& p = queue[n];
       int64_t units = 0;
       const int per = item_units(p, &units);
