@@ -19,8 +19,9 @@
 //    (0 auto, 1 vector, 2 merge / LDS window, 3 COO atomics, 4 / 5 / 6 CSR
 //    column panels: 8 / 16 / 4);
 //  * errors throw lespmv::Error instead of calling exit();
-//  * IndexType may be int or long long; long long matrices must fit 32-bit
-//    indices on the device.
+//  * IndexType may be int or long long; long long matrices keep 32-bit row
+//    and column indices on the device, and beyond 2^31 - 1 nonzeros 64-bit
+//    row offsets, which run the CSR kernels only.
 #pragma once
 
 #include <malloc.h>
@@ -229,11 +230,19 @@ std::shared_ptr<Handle> upload(const CSR_Matrix<I, V>& csr) {
                             reinterpret_cast<const int32_t*>(csr.col_index), csr.values, 0),
           "lespmv_create_csr");
   } else {
-    if (nnz >= INT32_MAX || m >= INT32_MAX || n >= INT32_MAX)
-      throw Error(LESPMV_ERR_UNSUPPORTED, "matrix does not fit 32-bit device indices");
-    std::vector<int32_t> rp(csr.row_offset, csr.row_offset + m + 1), ci(csr.col_index, csr.col_index + nnz);
-    check(lespmv_create_csr(&h->h, dtype_of<V>(), m, n, nnz, rp.data(), ci.data(), csr.values, 0),
-          "lespmv_create_csr");
+    if (m >= INT32_MAX || n >= INT32_MAX)
+      throw Error(LESPMV_ERR_UNSUPPORTED, "rows and columns must fit 32-bit device indices");
+    std::vector<int32_t> ci(csr.col_index, csr.col_index + nnz);
+    if (nnz >= INT32_MAX) {  // 64-bit row offsets on the device: CSR SpMV / SpMM only
+      if (sizeof(I) != sizeof(int64_t)) throw Error(LESPMV_ERR_UNSUPPORTED, "64-bit row offsets need a 64-bit IndexType");
+      check(lespmv_create_csr64(&h->h, dtype_of<V>(), m, n, nnz, reinterpret_cast<const int64_t*>(csr.row_offset),
+                                ci.data(), csr.values, 0),
+            "lespmv_create_csr64");
+    } else {
+      std::vector<int32_t> rp(csr.row_offset, csr.row_offset + m + 1);
+      check(lespmv_create_csr(&h->h, dtype_of<V>(), m, n, nnz, rp.data(), ci.data(), csr.values, 0),
+            "lespmv_create_csr");
+    }
   }
   return h;
 }

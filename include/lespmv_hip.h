@@ -140,6 +140,15 @@ int lespmv_default_params(int fmt, int dtype, int reference_layout, lespmv_param
  * on_device = 1 means the three pointers are device pointers. */
 int lespmv_create_csr(lespmv_matrix_t* out, int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* row_offset,
                       const int32_t* col_index, const void* values, int on_device);
+/* The same with 64-bit row offsets, for matrices beyond 2^31 - 1 nonzeros
+ * (CSR_Matrix<long long, V>, src/spmv_csr.cpp:187-193 instantiates it; rows
+ * and columns stay below 2^31, column indices are 32-bit). The matrix is held
+ * as row blocks of fewer than 2^31 nonzeros with 32-bit offsets each. Such a
+ * handle runs lespmv_spmv / lespmv_spmm of LESPMV_CSR (every CSR variant, block
+ * by block), their timing, lespmv_matrix_info and the byte counts; every other
+ * call on it returns LESPMV_ERR_UNSUPPORTED. */
+int lespmv_create_csr64(lespmv_matrix_t* out, int dtype, int64_t m, int64_t n, int64_t nnz, const int64_t* row_offset,
+                        const int32_t* col_index, const void* values, int on_device);
 int lespmv_destroy(lespmv_matrix_t A);
 int lespmv_matrix_info(lespmv_matrix_t A, int64_t* m, int64_t* n, int64_t* nnz, int32_t* dtype);
 

@@ -25,9 +25,18 @@ int guarded(F&& f) {
   return guarded_call(std::forward<F>(f));
 }
 
-Matrix& mat(lespmv_matrix_t h) {
+// Any handle; entry points that take 64-bit-offset handles (CSR SpMV / SpMM,
+// their timing, sizes) use it.
+Matrix& mat_any(lespmv_matrix_t h) {
   LSP_REQUIRE(h != nullptr, LESPMV_ERR_ARG, "null matrix handle");
   return *reinterpret_cast<Matrix*>(h);
+}
+
+// A handle with 32-bit row offsets: every other entry point.
+Matrix& mat(lespmv_matrix_t h) {
+  Matrix& A = mat_any(h);
+  LSP_REQUIRE(!A.idx64, LESPMV_ERR_UNSUPPORTED, "a handle with 64-bit row offsets runs CSR SpMV and SpMM only");
+  return A;
 }
 
 void check_fmt(int fmt) {
@@ -81,7 +90,13 @@ void dispatch_spmv(Matrix& A, int fmt, int variant, double alpha, const void* x,
 void run_spmv(Matrix& A, int fmt, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
   require_format(A, fmt);
   StreamOrder order(A, s);
-  dispatch_spmv(A, fmt, variant, alpha, x, beta, y, s);
+  if (A.idx64) {
+    for (size_t b = 0; b < A.blocks.size(); b++)
+      csr_spmv(*A.blocks[b], variant, alpha, x, beta,
+               static_cast<char*>(y) + A.block_row[b] * (int64_t)vsize(A.dtype), s);
+  } else {
+    dispatch_spmv(A, fmt, variant, alpha, x, beta, y, s);
+  }
   order.done();
 }
 
@@ -91,6 +106,13 @@ void run_spmm(Matrix& A, int fmt, int variant, int k, double alpha, const void* 
   LSP_REQUIRE(k >= 0 && k <= 64, LESPMV_ERR_UNSUPPORTED, "SpMM supports 0 <= k <= 64");
   LSP_REQUIRE(ldx >= k && ldy >= k, LESPMV_ERR_ARG, "leading dimensions must be >= k");
   StreamOrder order(A, s);
+  if (A.idx64) {
+    for (size_t b = 0; b < A.blocks.size(); b++)
+      csr_spmm(*A.blocks[b], variant, k, alpha, X, ldx, beta,
+               static_cast<char*>(Y) + A.block_row[b] * ldy * (int64_t)vsize(A.dtype), ldy, s);
+    order.done();
+    return;
+  }
   switch (fmt) {
     case LESPMV_CSR: csr_spmm(A, variant, k, alpha, X, ldx, beta, Y, ldy, s); break;
     case LESPMV_COO: coo_spmm(A, variant, k, alpha, X, ldx, beta, Y, ldy, s); break;
@@ -112,7 +134,7 @@ int64_t effective_bytes(Matrix& A, int fmt, int k, bool beta_nz) {
   const int64_t sv = (int64_t)vsize(A.dtype), si = 4;
   const int64_t vec = (int64_t)k * (A.n + A.m * (beta_nz ? 2 : 1)) * sv;
   switch (fmt) {
-    case LESPMV_CSR: return A.nnz * (sv + si) + (A.m + 1) * si + vec;
+    case LESPMV_CSR: return A.nnz * (sv + si) + (A.m + 1) * (A.idx64 ? 8 : si) + vec;
     case LESPMV_COO: return A.nnz * (sv + 2 * si) + vec;
     case LESPMV_ELL:
     case LESPMV_SELL:
@@ -245,6 +267,41 @@ void export_array(Matrix& A, int fmt, const std::string& name, std::vector<unsig
 
 using namespace lsp;
 
+namespace {
+
+// An empty handle: device, stream and events.
+std::unique_ptr<Matrix> new_matrix(int dtype, int64_t m, int64_t n, int64_t nnz) {
+  auto A = std::make_unique<Matrix>();
+  A->dtype = dtype;
+  A->m = m;
+  A->n = n;
+  A->nnz = nnz;
+  LSP_HIP(hipGetDevice(&A->device));
+  LSP_HIP(hipStreamCreate(&A->stream));
+  LSP_HIP(hipEventCreateWithFlags(&A->ev_switch, hipEventDisableTiming));
+  LSP_HIP(hipEventCreate(&A->ev0));
+  LSP_HIP(hipEventCreate(&A->ev1));
+  return A;
+}
+
+// A 32-bit CSR handle with its arrays copied in (kind: from host or device).
+std::unique_ptr<Matrix> make_csr(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* row_offset,
+                                 const int32_t* col_index, const void* values, hipMemcpyKind kind) {
+  auto A = new_matrix(dtype, m, n, nnz);
+  A->rp.alloc(sizeof(int32_t) * (m + 1));
+  A->ci.alloc(sizeof(int32_t) * nnz);
+  A->val.alloc(vsize(dtype) * nnz);
+  LSP_HIP(hipMemcpyAsync(A->rp.p, row_offset, sizeof(int32_t) * (m + 1), kind, A->stream));
+  if (nnz) {
+    LSP_HIP(hipMemcpyAsync(A->ci.p, col_index, sizeof(int32_t) * nnz, kind, A->stream));
+    LSP_HIP(hipMemcpyAsync(A->val.p, values, vsize(dtype) * nnz, kind, A->stream));
+  }
+  LSP_HIP(hipStreamSynchronize(A->stream));
+  return A;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* lespmv_last_error(void) { return g_last_error.c_str(); }
@@ -305,26 +362,50 @@ int lespmv_create_csr(lespmv_matrix_t* out, int dtype, int64_t m, int64_t n, int
     LSP_REQUIRE(m < (int64_t)INT32_MAX && n < (int64_t)INT32_MAX && nnz < (int64_t)INT32_MAX,
                 LESPMV_ERR_UNSUPPORTED, "sizes must fit 32-bit indices");
     LSP_REQUIRE(row_offset && (nnz == 0 || (col_index && values)), LESPMV_ERR_ARG, "null CSR array");
-    auto A = std::make_unique<Matrix>();
-    A->dtype = dtype;
-    A->m = m;
-    A->n = n;
-    A->nnz = nnz;
-    LSP_HIP(hipGetDevice(&A->device));
-    LSP_HIP(hipStreamCreate(&A->stream));
-    LSP_HIP(hipEventCreateWithFlags(&A->ev_switch, hipEventDisableTiming));
-    LSP_HIP(hipEventCreate(&A->ev0));
-    LSP_HIP(hipEventCreate(&A->ev1));
-    A->rp.alloc(sizeof(int32_t) * (m + 1));
-    A->ci.alloc(sizeof(int32_t) * nnz);
-    A->val.alloc(vsize(dtype) * nnz);
+    auto A = make_csr(dtype, m, n, nnz, row_offset, col_index, values,
+                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+    *out = reinterpret_cast<lespmv_matrix_t>(A.release());
+  });
+}
+
+int lespmv_create_csr64(lespmv_matrix_t* out, int dtype, int64_t m, int64_t n, int64_t nnz, const int64_t* row_offset,
+                        const int32_t* col_index, const void* values, int on_device) {
+  return guarded([&] {
+    LSP_REQUIRE(out, LESPMV_ERR_ARG, "null output handle");
+    LSP_REQUIRE(dtype == LESPMV_F32 || dtype == LESPMV_F64, LESPMV_ERR_ARG, "dtype must be F32 or F64");
+    LSP_REQUIRE(m >= 0 && n >= 0 && nnz >= 0, LESPMV_ERR_ARG, "negative size");
+    LSP_REQUIRE(m < (int64_t)INT32_MAX && n < (int64_t)INT32_MAX, LESPMV_ERR_UNSUPPORTED,
+                "rows and columns must fit 32-bit indices");
+    LSP_REQUIRE(row_offset && (nnz == 0 || (col_index && values)), LESPMV_ERR_ARG, "null CSR array");
     const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    LSP_HIP(hipMemcpyAsync(A->rp.p, row_offset, sizeof(int32_t) * (m + 1), kind, A->stream));
-    if (nnz) {
-      LSP_HIP(hipMemcpyAsync(A->ci.p, col_index, sizeof(int32_t) * nnz, kind, A->stream));
-      LSP_HIP(hipMemcpyAsync(A->val.p, values, vsize(dtype) * nnz, kind, A->stream));
+    std::vector<int64_t> rp64;
+    const int64_t* rp = row_offset;
+    if (on_device) {
+      rp64.resize(m + 1);
+      LSP_HIP(hipMemcpy(rp64.data(), row_offset, sizeof(int64_t) * (m + 1), hipMemcpyDeviceToHost));
+      rp = rp64.data();
     }
-    LSP_HIP(hipStreamSynchronize(A->stream));
+    LSP_REQUIRE(rp[0] == 0 && rp[m] == nnz, LESPMV_ERR_ARG, "row offsets must run from 0 to nnz");
+    auto A = new_matrix(dtype, m, n, nnz);
+    A->idx64 = true;
+    // row blocks of at most INT32_MAX - 1 nonzeros, each with 32-bit offsets from its first entry
+    int64_t r0 = 0;
+    A->block_row.push_back(0);
+    std::vector<int32_t> local;
+    while (r0 < m || A->blocks.empty()) {
+      const int64_t limit = rp[r0] + (int64_t)INT32_MAX - 1;
+      const int64_t r1 = std::max<int64_t>(std::upper_bound(rp + r0, rp + m + 1, limit) - rp - 1, r0);
+      LSP_REQUIRE(r1 > r0 || m == 0, LESPMV_ERR_UNSUPPORTED, "a row holds 2^31 - 1 or more nonzeros");
+      local.resize(r1 - r0 + 1);
+      for (int64_t r = r0; r <= r1; r++) local[r - r0] = (int32_t)(rp[r] - rp[r0]);
+      const int64_t e0 = rp[r0], cnt = rp[r1] - rp[r0];
+      A->blocks.push_back(make_csr(dtype, r1 - r0, n, cnt, local.data(), col_index ? col_index + e0 : nullptr,
+                                   values ? static_cast<const char*>(values) + e0 * (int64_t)vsize(dtype) : nullptr,
+                                   kind));
+      A->block_row.push_back(r1);
+      r0 = r1;
+      if (m == 0) break;
+    }
     *out = reinterpret_cast<lespmv_matrix_t>(A.release());
   });
 }
@@ -342,7 +423,7 @@ int lespmv_destroy(lespmv_matrix_t h) {
 
 int lespmv_matrix_info(lespmv_matrix_t h, int64_t* m, int64_t* n, int64_t* nnz, int32_t* dtype) {
   return guarded([&] {
-    Matrix& A = mat(h);
+    Matrix& A = mat_any(h);
     if (m) *m = A.m;
     if (n) *n = A.n;
     if (nnz) *nnz = A.nnz;
@@ -445,7 +526,7 @@ int lespmv_convert_ms(lespmv_matrix_t h, int fmt, double* ms) {
 int lespmv_spmv(lespmv_matrix_t h, int fmt, int variant, double alpha, const void* x, double beta, void* y,
                 lespmv_stream_t stream) {
   return guarded([&] {
-    Matrix& A = mat(h);
+    Matrix& A = mat_any(h);
     check_fmt(fmt);
     LSP_REQUIRE((x || A.n == 0) && (y || A.m == 0), LESPMV_ERR_ARG, "null vector");
     run_spmv(A, fmt, variant, alpha, x, beta, y, pick(A, stream));
@@ -455,7 +536,7 @@ int lespmv_spmv(lespmv_matrix_t h, int fmt, int variant, double alpha, const voi
 int lespmv_spmm(lespmv_matrix_t h, int fmt, int variant, int32_t k, double alpha, const void* X, int64_t ldx,
                 double beta, void* Y, int64_t ldy, lespmv_stream_t stream) {
   return guarded([&] {
-    Matrix& A = mat(h);
+    Matrix& A = mat_any(h);
     check_fmt(fmt);
     run_spmm(A, fmt, variant, k, alpha, X, ldx, beta, Y, ldy, pick(A, stream));
   });
@@ -464,7 +545,7 @@ int lespmv_spmm(lespmv_matrix_t h, int fmt, int variant, int32_t k, double alpha
 int lespmv_time_spmv(lespmv_matrix_t h, int fmt, int variant, double alpha, const void* x, double beta, void* y,
                      int32_t iters, double* ms_per_call) {
   return guarded([&] {
-    Matrix& A = mat(h);
+    Matrix& A = mat_any(h);
     check_fmt(fmt);
     LSP_REQUIRE(iters > 0 && ms_per_call, LESPMV_ERR_ARG, "iters must be > 0");
     hipStream_t s = A.stream;
@@ -509,7 +590,7 @@ int lespmv_time_spmv_rotating(const lespmv_matrix_t* handles, int32_t count, int
 int lespmv_time_spmm(lespmv_matrix_t h, int fmt, int variant, int32_t k, double alpha, const void* X, int64_t ldx,
                      double beta, void* Y, int64_t ldy, int32_t iters, double* ms_per_call) {
   return guarded([&] {
-    Matrix& A = mat(h);
+    Matrix& A = mat_any(h);
     check_fmt(fmt);
     LSP_REQUIRE(iters > 0 && ms_per_call, LESPMV_ERR_ARG, "iters must be > 0");
     hipStream_t s = A.stream;
@@ -525,7 +606,7 @@ int lespmv_time_spmm(lespmv_matrix_t h, int fmt, int variant, int32_t k, double 
 
 int lespmv_effective_bytes(lespmv_matrix_t h, int fmt, int32_t k, int beta_nonzero, int64_t* bytes) {
   return guarded([&] {
-    Matrix& A = mat(h);
+    Matrix& A = mat_any(h);
     check_fmt(fmt);
     require_format(A, fmt);
     *bytes = effective_bytes(A, fmt, k < 1 ? 1 : k, beta_nonzero != 0);
@@ -543,11 +624,13 @@ int lespmv_reference_bytes(lespmv_matrix_t h, int fmt, int64_t* bytes) {
 
 int lespmv_format_bytes(lespmv_matrix_t h, int fmt, int64_t* bytes) {
   return guarded([&] {
-    Matrix& A = mat(h);
+    Matrix& A = mat_any(h);
     check_fmt(fmt);
     require_format(A, fmt);
     *bytes = fmt == LESPMV_CSR ? (int64_t)(A.rp.bytes + A.ci.bytes + A.val.bytes) + csr_aux_bytes(A)
                                : A.fmt[fmt]->device_bytes();
+    for (const auto& b : A.blocks)  // the row blocks of a handle given with 64-bit offsets
+      *bytes += (int64_t)(b->rp.bytes + b->ci.bytes + b->val.bytes) + csr_aux_bytes(*b);
   });
 }
 

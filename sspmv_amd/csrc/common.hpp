@@ -189,6 +189,14 @@ struct Matrix {
   hipEvent_t ev_switch = nullptr;
   double prepare_ms = 0;  // device time of the last CSR prepare (panel layout)
   DBuf rp, ci, val;  // CSR: int32[m+1], int32[nnz], V[nnz]
+  // A matrix given with 64-bit row offsets (lespmv_create_csr64, beyond
+  // 2^31 - 1 nonzeros) is held as row blocks of fewer than 2^31 nonzeros,
+  // each an ordinary handle with 32-bit offsets relative to its first entry;
+  // block b holds rows [block_row[b], block_row[b + 1]). Such a handle runs
+  // CSR SpMV / SpMM only.
+  bool idx64 = false;
+  std::vector<std::unique_ptr<Matrix>> blocks;
+  std::vector<int64_t> block_row;
   // int32[m]: the rows the CSR row engine (SpMM variants 16..22) takes, by
   // position; empty = natural order. Set by the serving layer for matrices
   // whose rows differ in length; results do not depend on it.

@@ -783,6 +783,7 @@ extern "C" int lespmv_features(lespmv_matrix_t h, int32_t flags, lespmv_features
   return lsp::guarded_call([&] {
     LSP_REQUIRE(h && out, LESPMV_ERR_ARG, "null argument");
     Matrix& A = *reinterpret_cast<Matrix*>(h);
+    LSP_REQUIRE(!A.idx64, LESPMV_ERR_UNSUPPORTED, "a handle with 64-bit row offsets runs CSR SpMV and SpMM only");
     if (A.dtype == LESPMV_F64) compute<double>(A, flags, *out, images);
     else compute<float>(A, flags, *out, images);
   });
@@ -809,6 +810,7 @@ extern "C" int lespmv_block_counts(lespmv_matrix_t h, const int32_t* dims, int32
   return lsp::guarded_call([&] {
     LSP_REQUIRE(h && (n == 0 || (dims && nnzb)), LESPMV_ERR_ARG, "null argument");
     Matrix& A = *reinterpret_cast<Matrix*>(h);
+    LSP_REQUIRE(!A.idx64, LESPMV_ERR_UNSUPPORTED, "a handle with 64-bit row offsets runs CSR SpMV and SpMM only");
     hipStream_t s = A.stream;
     DBuf keys, out;
     keys.alloc(sizeof(u64) * std::max<int64_t>(A.nnz, 1));

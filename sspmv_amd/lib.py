@@ -176,6 +176,7 @@ def load() -> C.CDLL:
         "lespmv_selector_inputs": ([vp, i32, vp, vp, C.POINTER(FeaturesC)], C.c_int),
         "lespmv_selector_decode": ([vp, vp, C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "lespmv_select": ([vp, vp, i32, C.POINTER(TuneResultC), C.POINTER(i32)], C.c_int),
+        "lespmv_create_csr64": ([C.POINTER(vp), C.c_int, i64, i64, i64, vp, vp, vp, C.c_int], C.c_int),
         "lespmv_server_create": ([C.POINTER(vp), i32, i64, i32, i32], C.c_int),
         "lespmv_server_destroy": ([vp], C.c_int),
         "lespmv_server_add_csr": ([vp, i64, i64, i64, vp, vp, vp, i32, C.POINTER(i32)], C.c_int),
@@ -357,6 +358,21 @@ class Matrix:
     @classmethod
     def from_host(cls, A) -> "Matrix":
         return cls(A.m, A.n, A.rp, A.ci, A.v)
+
+    @classmethod
+    def csr64(cls, m: int, n: int, rp: np.ndarray, ci: np.ndarray, v: np.ndarray) -> "Matrix":
+        """A handle with 64-bit row offsets (lespmv_create_csr64): CSR SpMV / SpMM only."""
+        M = cls.__new__(cls)
+        M.h = C.c_void_p()
+        rp = np.ascontiguousarray(rp, dtype=np.int64)
+        ci = np.ascontiguousarray(ci, dtype=np.int32)
+        v = np.ascontiguousarray(v)
+        M.dtype = v.dtype
+        M.m, M.n, M.nnz = int(m), int(n), int(ci.shape[0])
+        check(load().lespmv_create_csr64(C.byref(M.h), dtype_id(v.dtype), M.m, M.n, M.nnz,
+                                         rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                                         v.ctypes.data_as(C.c_void_p), 0))
+        return M
 
     @classmethod
     def borrowed(cls, h, m: int, n: int, nnz: int, dtype, owner=None) -> "Matrix":
