@@ -27,16 +27,19 @@ def test_csr64_matches_oracle(gpu, oracle, dt):
         x = rng.uniform(-1, 1, A.n).astype(dt)
         y0 = rng.uniform(-1, 1, A.m).astype(dt)
         dx, dy = lib.DeviceArray.from_numpy(x), lib.DeviceArray.from_numpy(y0)
-        M.spmv(lib.CSR, 0.8, dx, 0.7, dy)
         ref = y0.astype(np.float64)
         oracle.spmv_csr(A.m, A.n, A.rp, A.ci, A.v.astype(np.float64), 0.8, x.astype(np.float64), 0.7, ref)
-        assert rel2(dy.numpy(), ref) <= TOL[np.dtype(dt)], (A.m, A.nnz)
-        # SpMM with k = 3, ldx = 5, ldy = 4: each column, padding untouched
+        # every CSR variant runs on the row blocks: auto, vector, merge, vector with 4 lanes a row
+        for variant in (lib.VARIANT_AUTO, lib.VARIANT_VECTOR, lib.VARIANT_MERGE, 18):
+            dy = lib.DeviceArray.from_numpy(y0)
+            M.spmv(lib.CSR, 0.8, dx, 0.7, dy, variant=variant)
+            assert rel2(dy.numpy(), ref) <= TOL[np.dtype(dt)], (A.m, A.nnz, variant)
+        # SpMM (the row engine, 2 lanes) with k = 3, ldx = 5, ldy = 4: each column, padding untouched
         k, ldx, ldy = 3, 5, 4
         X = rng.uniform(-1, 1, (A.n, ldx)).astype(dt)
         Y0 = rng.uniform(-1, 1, (A.m, ldy)).astype(dt)
         dX, dY = lib.DeviceArray.from_numpy(X), lib.DeviceArray.from_numpy(Y0)
-        M.spmm(lib.CSR, k, -0.5, dX, ldx, 1.5, dY, ldy)
+        M.spmm(lib.CSR, k, -0.5, dX, ldx, 1.5, dY, ldy, variant=17)
         Y = dY.numpy()
         np.testing.assert_array_equal(Y[:, k:], Y0[:, k:])
         for j in range(k):
