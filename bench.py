@@ -54,7 +54,7 @@ def kernels(lib, section):
     """(label, fmt, variant, params) measured per section."""
     sell = lambda C, s: lib.make_params(chunk=C, sigma=s, alignment=1)
     csr = [("CSR-auto", lib.CSR, lib.VARIANT_AUTO, None), ("CSR-vector", lib.CSR, lib.VARIANT_VECTOR, None),
-           ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None)]
+           ("CSR-merge", lib.CSR, lib.VARIANT_MERGE, None), ("CSR-rowtiles", lib.CSR, lib.VARIANT_ROW_TILES, None)]
     if section.startswith("stencil27"):
         return csr + [("SELL-64-sigma4096", lib.SELL_C_SIGMA, 0, sell(64, 4096)),
                       ("DIA", lib.DIA, lib.VARIANT_VECTOR, lib.make_params(alignment=1, max_diags=10240)),

@@ -87,9 +87,11 @@ enum {
   LESPMV_VARIANT_PANEL = 4,     /* CSR: 8 column panels, one per XCD (x slice held in that XCD's L2),
                                    merge-path per panel, partial y reduced in panel order */
   LESPMV_VARIANT_PANEL16 = 5,   /* CSR: the same with 16 panels (two per XCD) */
-  LESPMV_VARIANT_PANEL4 = 6     /* CSR: 4 panels, each run by two XCDs; the 4 MB L2 of an XCD holds an
+  LESPMV_VARIANT_PANEL4 = 6,    /* CSR: 4 panels, each run by two XCDs; the 4 MB L2 of an XCD holds an
                                    x slice when x is about 16 MB (FP32 x of 4M columns), as 8 panels
                                    do for 32 MB (FP64) */
+  LESPMV_VARIANT_ROW_TILES = 7  /* CSR: merge-path tiles cut at row starts, so no row spans two tiles and
+                                   no carry fix-up pass runs (rows of at most 511 entries; else MERGE) */
 };
 
 /* Conversion parameters. Fields a format does not use are ignored. */

@@ -163,6 +163,9 @@ struct CsrMeta {
   int32_t auto_variant = 1;  // tuned CSR kernel
   double far_frac = 0;       // far x lines per entry (entries > 2 MB of x off the diagonal opening a new line)
   int64_t merge_tiles = 0;
+  // tiles cut at row starts (variant ROW_TILES): no carries between tiles
+  int64_t row_tiles = 0;
+  DBuf row_coord, row_carry_row, row_carry_val, row_head;
   DBuf merge_coord;          // int32[2*(tiles+1)] (row, nnz) at each tile start
   DBuf carry_row;            // int32[tiles]  row open at the tile end (-1: none)
   DBuf carry_val;            // V[tiles]      its partial sum inside the tile

@@ -105,6 +105,20 @@ __global__ void merge_partition_kernel(int64_t tiles, int64_t m, int64_t nnz, co
   coord[2 * t + 1] = (int)(d - r);
 }
 
+// Tiles cut at row starts: the merge-path cut of tile t (every `items`
+// diagonals) moved back to the start of the row it falls in. A tile then
+// holds whole rows only, at most items + the longest row + 1 merge-path items.
+__global__ void row_partition_kernel(int64_t tiles, int64_t m, int64_t nnz, const int* __restrict__ rp,
+                                     int* __restrict__ coord, int64_t items) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > tiles) return;
+  int64_t d = t * items;
+  if (d > m + nnz) d = m + nnz;
+  const int64_t r = merge_search(d, rp, m, nnz);
+  coord[2 * t] = (int)r;
+  coord[2 * t + 1] = rp[r];
+}
+
 template <class V, int BT = kBlock>
 __device__ V block_sum(V v, V* red) {
   v = group_sum<64>(v);
@@ -574,7 +588,7 @@ template <class V>
 void merge_launch(int64_t m, int64_t n, int64_t tiles, const int* rp, const int* ci, const V* val, const void* x,
                   void* y,
                   double alpha, double beta, const int* coord, int* carry_row, V* carry_val, V* head_val,
-                  hipStream_t s) {
+                  hipStream_t s, bool fixup = true) {
   if (tiles == 0) return;
   // a matrix beyond the 256 MB last-level cache is streamed with
   // non-temporal loads, so its arrays do not push x out of the caches
@@ -588,6 +602,7 @@ void merge_launch(int64_t m, int64_t n, int64_t tiles, const int* rp, const int*
                      static_cast<const V*>(x), static_cast<V*>(y), (V)alpha, (V)beta, coord, carry_row, carry_val,
                      head_val);
   LSP_LAUNCHED();
+  if (!fixup) return;  // tiles cut at row starts: no row continues into the next tile
   hipLaunchKernelGGL((csr_merge_fixup_kernel<V>), grid1d(tiles, kBlock), dim3(kBlock), 0, s, tiles, coord, rp,
                      carry_row, carry_val, head_val, static_cast<V*>(y), (V)alpha, (V)beta);
   LSP_LAUNCHED();
@@ -901,6 +916,13 @@ void csr_analyse(Matrix& A, hipStream_t s) {
     mt.auto_variant = LESPMV_VARIANT_MERGE;
   else
     mt.auto_variant = LESPMV_VARIANT_VECTOR;
+  // merge tiles cut at row starts when every row fits a quarter tile: the
+  // same tile kernel without the carry fix-up pass, never slower than the
+  // merge path where both apply (27-point stencil 160^3 FP64 278 against
+  // 280 us, banded rows of 17 on 2M rows 72 against 74, power-law rows of up
+  // to 400 entries 28.5 against 33.4)
+  if (mt.auto_variant == LESPMV_VARIANT_MERGE && mt.max_row + 1 <= kMergeItems / 4)
+    mt.auto_variant = LESPMV_VARIANT_ROW_TILES;
   const int64_t items = A.m + A.nnz;
   mt.merge_tiles = ceil_div(items, kMergeItems);
   mt.merge_coord.alloc(sizeof(int) * 2 * (mt.merge_tiles + 1));
@@ -915,9 +937,13 @@ void csr_analyse(Matrix& A, hipStream_t s) {
   mt.analysed = true;
 }
 
+bool row_tiles_fit(const Matrix& A);
+void row_tiles_prepare(Matrix& A, hipStream_t s);
+
 void csr_prepare(Matrix& A, int variant, hipStream_t s) {
   if (!A.meta.analysed) csr_analyse(A, s);
   if (variant == LESPMV_VARIANT_AUTO) variant = A.meta.auto_variant;
+  if (variant == LESPMV_VARIANT_ROW_TILES && A.m && row_tiles_fit(A) && !A.meta.row_coord.p) row_tiles_prepare(A, s);
   const int P = panels_of(variant);
   if (P && A.m && A.meta.panels != P) {
     if (A.dtype == LESPMV_F64) build_panels<double>(A, P, s);
@@ -936,17 +962,51 @@ void csr_release_aux(Matrix& A) {
 int64_t csr_aux_bytes(const Matrix& A) {
   const CsrMeta& mt = A.meta;
   int64_t b = 0;
-  for (const DBuf* d : {&mt.merge_coord, &mt.carry_row, &mt.carry_val, &mt.head_val, &mt.p_nzoff, &mt.p_rp,
+  for (const DBuf* d : {&mt.merge_coord, &mt.carry_row, &mt.carry_val, &mt.head_val, &mt.row_coord,
+                        &mt.row_carry_row, &mt.row_carry_val, &mt.row_head, &mt.p_nzoff, &mt.p_rp,
                         &mt.p_ci, &mt.p_val, &mt.p_tiles, &mt.p_coord, &mt.p_carry_row, &mt.p_carry_val, &mt.p_head,
                         &mt.p_partial, &mt.p_ends, &mt.p_endoff, &mt.p_ci16, &mt.p_cihi})
     b += (int64_t)d->bytes;
   return b;
 }
 
+// The row-aligned partition of variant ROW_TILES (rows of at most
+// kMergeItems / 4 - 1 entries): tiles of kMergeItems - (longest row + 1)
+// nominal merge-path items, cut back to row starts.
+bool row_tiles_fit(const Matrix& A) { return A.meta.max_row + 1 <= kMergeItems / 4; }
+
+void row_tiles_prepare(Matrix& A, hipStream_t s) {
+  CsrMeta& mt = A.meta;
+  const int64_t items = kMergeItems - (mt.max_row + 1);
+  mt.row_tiles = ceil_div(A.m + A.nnz, items);
+  mt.row_coord.alloc(sizeof(int) * 2 * (mt.row_tiles + 1));
+  mt.row_carry_row.alloc(sizeof(int) * std::max<int64_t>(mt.row_tiles, 1));
+  mt.row_carry_val.alloc(vsize(A.dtype) * std::max<int64_t>(mt.row_tiles, 1));
+  mt.row_head.alloc(vsize(A.dtype) * std::max<int64_t>(mt.row_tiles, 1));
+  hipLaunchKernelGGL(row_partition_kernel, grid1d(mt.row_tiles + 1, kBlock), dim3(kBlock), 0, s, mt.row_tiles, A.m,
+                     A.nnz, A.rp.as<int>(), mt.row_coord.as<int>(), items);
+  LSP_LAUNCHED();
+  LSP_HIP(hipStreamSynchronize(s));  // complete before any call (on any stream) reads it
+}
+
+template <class V>
+void row_tiles_run(Matrix& A, double alpha, const void* x, double beta, void* y, hipStream_t s) {
+  CsrMeta& mt = A.meta;
+  if (!mt.row_coord.p) row_tiles_prepare(A, s);
+  merge_launch<V>(A.m, A.n, mt.row_tiles, A.rp.as<int>(), A.ci.as<int>(), A.val.as<V>(), x, y, alpha, beta,
+                  mt.row_coord.as<int>(), mt.row_carry_row.as<int>(), mt.row_carry_val.as<V>(), mt.row_head.as<V>(),
+                  s, false);
+}
+
 void csr_spmv(Matrix& A, int variant, double alpha, const void* x, double beta, void* y, hipStream_t s) {
   if (A.m == 0) return;
   if (!A.meta.analysed) csr_analyse(A, s);
   if (variant == LESPMV_VARIANT_AUTO) variant = A.meta.auto_variant;
+  if (variant == LESPMV_VARIANT_ROW_TILES) {
+    if (!row_tiles_fit(A)) variant = LESPMV_VARIANT_MERGE;
+    else if (A.dtype == LESPMV_F64) return row_tiles_run<double>(A, alpha, x, beta, y, s);
+    else return row_tiles_run<float>(A, alpha, x, beta, y, s);
+  }
   if (variant == LESPMV_VARIANT_MERGE) {
     if (A.dtype == LESPMV_F64) merge_run<double>(A, alpha, x, beta, y, s);
     else merge_run<float>(A, alpha, x, beta, y, s);

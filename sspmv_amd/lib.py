@@ -21,6 +21,7 @@ FORMAT_NAMES = ["COO", "CSR", "DIA", "ELL", "S-ELL", "SELL-c-sigma", "SELL-c-R",
 F32, F64 = 0, 1
 VARIANT_AUTO, VARIANT_VECTOR, VARIANT_MERGE, VARIANT_ATOMIC, VARIANT_PANEL, VARIANT_PANEL16 = 0, 1, 2, 3, 4, 5
 VARIANT_PANEL4 = 6
+VARIANT_ROW_TILES = 7
 GEN_LAPLACE2D, GEN_STENCIL27, GEN_LOGNORMAL, GEN_BLOCKRAND, GEN_BANDED, GEN_POWERLAW, GEN_BLOCKDIAG, GEN_UNIFORM = \
     range(8)
 

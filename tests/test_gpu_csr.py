@@ -12,7 +12,8 @@ from tests.matrices import HostCsr, random_csr, suite
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [0, 1, 2, 4, 5, 6, 17, 18, 19, 20, 21, 22]  # auto, vector(tuned), merge, panels 8 / 16 / 4, vector L=2..64
+# auto, vector (tuned), merge, panels 8 / 16 / 4, merge tiles cut at row starts, vector L = 2..64
+VARIANTS = [0, 1, 2, 4, 5, 6, 7, 17, 18, 19, 20, 21, 22]
 
 
 def oracle_y(oracle, A, alpha, x, beta, y0):
@@ -74,9 +75,38 @@ def test_csr_edge_shapes(gpu, oracle):
         x = rng.uniform(-1, 1, A.n)
         y0 = rng.uniform(-1, 1, A.m)
         ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
-        for var in (1, 2, 4, 5, 6, 17, 19, 21, 22):
+        for var in (1, 2, 4, 5, 6, 7, 17, 19, 21, 22):
             y = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
             assert rel2(y, ref) <= 1e-12, (name, var)
+
+
+@pytest.mark.parametrize("longest", [511, 512])
+def test_row_tiles_longest_row(gpu, oracle, longest):
+    """Merge-path tiles cut at row starts take rows of up to 511 entries (a
+    tile of 2048 items holds its nominal 2048 - 512 items plus one whole
+    row); one row of 512 sends the call to the merge path with its fix-up.
+    Rows of the limit's length sit back to back, after runs of empty rows
+    and at the matrix end, so tile cuts land inside them."""
+    rng = np.random.default_rng(longest)
+    m, n = 6000, 9000
+    lens = rng.poisson(5.0, m)
+    lens[rng.random(m) < 0.3] = 0
+    lens[1000:1040] = longest
+    lens[3000:3400] = 0
+    lens[rng.choice(m, 25, replace=False)] = longest
+    lens[-3:] = longest
+    rp = np.zeros(m + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    ci = np.concatenate([np.sort(rng.choice(n, int(L), replace=False)) for L in lens]).astype(np.int32)
+    A = HostCsr(m, n, rp, ci, rng.uniform(-1, 1, int(rp[-1])))
+    M = gpu.Matrix.from_host(A)
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, m)
+    ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
+    for beta in (0.7, 0.0):
+        want = ref if beta else oracle_y(oracle, A, 0.8, x, 0.0, y0)
+        y = gpu_y(gpu, M, CSR, 0.8, x, beta, y0, gpu.VARIANT_ROW_TILES)
+        assert rel2(y, want) <= 1e-12, (longest, beta)
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
@@ -89,7 +119,7 @@ def test_csr_generated_configs(gpu, oracle, dt):
         x = gpu.fill_uniform(A.n, 4, dt)
         y0 = gpu.fill_uniform(A.m, 9, dt)
         ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
-        for var in (0, 1, 2, 4, 5, 6):
+        for var in (0, 1, 2, 4, 5, 6, 7):
             y = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
             err = rel2(y, ref)
             print(f"{np.dtype(dt).name} fam={fam} variant={var} rel2={err:.3e}")
@@ -168,8 +198,9 @@ def test_csr_merge_wide_rows(gpu, oracle, dt):
         x = rng.uniform(-1, 1, A.n).astype(dt)
         y0 = rng.uniform(-1, 1, A.m).astype(dt)
         ref = oracle_y(oracle, A, 0.8, x, 0.7, y0)
-        got = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, gpu.VARIANT_MERGE)
-        assert rel2(got, ref) <= TOL[np.dtype(dt)], (A.m, A.nnz)
+        for var in (gpu.VARIANT_MERGE, gpu.VARIANT_ROW_TILES):
+            got = gpu_y(gpu, M, CSR, 0.8, x, 0.7, y0, var)
+            assert rel2(got, ref) <= TOL[np.dtype(dt)], (A.m, A.nnz, var)
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
@@ -211,7 +242,7 @@ def test_scratch_calls_across_streams(gpu, oracle):
     s = C.c_void_p()
     gpu.check(L.lespmv_stream_create(C.byref(s)))
     try:
-        for var in (gpu.VARIANT_MERGE, gpu.VARIANT_PANEL, gpu.VARIANT_PANEL4):
+        for var in (gpu.VARIANT_MERGE, gpu.VARIANT_ROW_TILES, gpu.VARIANT_PANEL, gpu.VARIANT_PANEL4):
             M = gpu.Matrix.from_host(A)
             dx = gpu.DeviceArray.from_numpy(x)
             ys = [gpu.DeviceArray.from_numpy(y0) for _ in range(3)]
@@ -287,14 +318,15 @@ def test_auto_plan(gpu):
     """The CSR variant AUTO runs (lespmv_csr_plan): column panels when x spans
     16 MB or more with 0.3 far x lines per entry or more (random 2x2 blocks:
     half a line per entry), not for random 4x4 blocks (a quarter line per
-    entry), which at 285 MB and 4 entries a row take the merge path, as does a
-    27-point stencil beyond the 256 MB last-level cache; the vector kernel
-    for a stencil inside it."""
+    entry), which at 285 MB and 4 entries a row take the merge-path tiles (cut
+    at row starts, as every row fits a quarter tile), as does a 27-point
+    stencil beyond the 256 MB last-level cache; the vector kernel for a
+    stencil inside it."""
     cases = [
         (gpu.GEN_LOGNORMAL, [4194304, 4194304, 2.0, 1.0, 64, 0.3, 64], gpu.VARIANT_PANEL, (0.3, 1.0)),
         (gpu.GEN_BLOCKRAND, [2097152, 2097152, 2, 2, 4], gpu.VARIANT_PANEL, (0.4, 0.55)),
-        (gpu.GEN_BLOCKRAND, [1048576, 1048576, 4, 4, 1], gpu.VARIANT_MERGE, (0.2, 0.3)),  # 285 MB, 4 per row
-        (gpu.GEN_STENCIL27, [110, 26.0, -1.0], gpu.VARIANT_MERGE, (0.0, 0.01)),
+        (gpu.GEN_BLOCKRAND, [1048576, 1048576, 4, 4, 1], gpu.VARIANT_ROW_TILES, (0.2, 0.3)),  # 285 MB, 4 per row
+        (gpu.GEN_STENCIL27, [110, 26.0, -1.0], gpu.VARIANT_ROW_TILES, (0.0, 0.01)),
         (gpu.GEN_STENCIL27, [40, 26.0, -1.0], gpu.VARIANT_VECTOR, (0.0, 0.01)),
     ]
     for gen, params, want, (lo, hi) in cases:
