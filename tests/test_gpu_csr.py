@@ -165,7 +165,8 @@ def test_csr_spmm_row_engine(gpu, oracle, dt):
     worst = 0.0
     for A in mats:
         M = gpu.Matrix.from_host(A)
-        for k in (1, 2, 3, 4, 5, 8, 17, 32, 64):
+        # 17, 20, 32, 33 and 64: the row's 8 or 16 lanes share its index and value loads (lanes past k too)
+        for k in (1, 2, 3, 4, 5, 8, 17, 20, 32, 33, 64):
             for extra in (0, 1):
                 ld = k + extra
                 X = rng.uniform(-1, 1, (A.n, ld)).astype(dt)
