@@ -8,7 +8,7 @@ CPU timing spreadsheets. Here every step runs through the library:
   power-law rows, block-diagonal, uniform random, plus lognormal, 4x4-block
   random, 2-D and 3-D stencils), GPU features -> 128 selector inputs, and an
   exhaustive GPU timing of the 27-candidate grid (``lespmv_tune``) plus the
-  library's default CSR kernel. Exhaustive timing is the judge.
+  library's default CSR kernel. Exhaustive timing is the arbiter and the fallback.
 * ``labels``: per matrix the class of the fastest candidate, and for each class
   with parameters the fastest candidate of that class.
 * ``train``: the MLP (128 -> 128 -> 128 -> 32, six softmax heads) trained by the
